@@ -1,0 +1,183 @@
+/*
+ * svsflow.h — C ABI of libsvsflow.so, the MI355X (gfx950) implementation of
+ * the 3D TV-L1 optical flow and tricubic Catmull-Rom warping path of
+ * singleVentricleSegmentation.
+ *
+ * Conventions
+ *   - All pointers are device pointers (HIP global memory) unless a comment
+ *     says "host".  Tensors are FP32, C-contiguous, x fastest:
+ *       volume [Z,Y,X], vector field [Z,Y,X,C], matrix field [Z,Y,X,3,3],
+ *       CNN batch [B,C,Z,Y,X] with flow [B,Z,Y,X,3] (components x, y, z).
+ *   - `stream` is a hipStream_t (NULL = the legacy default stream).  Every call
+ *     only enqueues work; nothing synchronises the host unless stated.
+ *   - Outputs are fully overwritten; inputs are never modified.
+ *   - Return value: 0 = success, 1 = invalid argument, 2 = HIP error.  The
+ *     message of the last failure on the calling thread is svs_last_error().
+ *   - Enum values match the reference (svs/src/bindings.cpp:51-55, :99-104):
+ *       boundary       0 NEAREST, 1 MIRROR, 2 REFLECT
+ *       interpolation  0 NEAREST, 1 LINEAR, 2 CUBIC_HERMITESPLINE
+ *   - Mesh spacing h = L / (N - 1) computed in FP32 (coreDefines.h:72-75).
+ */
+#ifndef SVSFLOW_H
+#define SVSFLOW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Last error message of the calling thread ("" if none). */
+const char* svs_last_error(void);
+
+/* out[0:n] = value.  Helper used instead of hipMemset so graphs stay replayable. */
+int svs_fill(float* out, int64_t n, float value, void* stream);
+
+/* ------------------------------------------------ differential operators */
+
+/* Nabla3D_CD.forward (C = 1: [Z,Y,X] -> [Z,Y,X,3]) and .forwardVectorField
+ * (C = 3: [Z,Y,X,3] -> [Z,Y,X,3,3]); central differences, per-boundary face
+ * rules.  Replaces svs/src/differentialOps.h:83-98 (cuda_nabla3d_cd_forward,
+ * differentialOps.cu:1447; cuda_nabla3d_cd_forwardVectorField, :1588). */
+int svs_nabla3d_forward(const float* b, int C, int NZ, int NY, int NX, float hZ, float hY, float hX, int bdry,
+                        float* out, void* stream);
+
+/* Nabla3D_CD.backward (C = 1: [Z,Y,X,3] -> [Z,Y,X]) and .backwardVectorField
+ * (C = 3: [Z,Y,X,3,3] -> [Z,Y,X,3]); the exact adjoint of the forward
+ * operator.  Replaces differentialOps.h:87-98 (differentialOps.cu:1517, :1659). */
+int svs_nabla3d_backward(const float* p, int C, int NZ, int NY, int NX, float hZ, float hY, float hX, int bdry,
+                         float* out, void* stream);
+
+/* AnisotropicNabla3D.computeTangentVecs: G [N,3] -> s [N], n, t1, t2 [N,3].
+ * Replaces anisotropicDifferentialOps.h:69-72 (anisotropicDifferentialOps.cu:275). */
+int svs_aniso_tangents(const float* G, int64_t N, float alpha, float beta, float* s, float* n, float* t1, float* t2,
+                       void* stream);
+
+/* AnisotropicNabla3D.forwardVectorField / .backwardVectorField (the same
+ * symmetric operator): in [N,3,3] -> out [N,3,3].  Replaces
+ * anisotropicDifferentialOps.h:74-84 (anisotropicDifferentialOps.cu:322). */
+int svs_aniso_apply(const float* in, const float* s, const float* n, const float* t1, const float* t2, int64_t N,
+                    float* out, void* stream);
+
+/* ------------------------------------------------------------- proxes */
+
+/* TVL1OF3D_proxPrimal(primal [N,3], tau, lambda, rho [N], g [N,3]) -> [N,3].
+ * Replaces opticalFlowOps.h:103-113 (opticalFlowOps.cu:421). */
+int svs_prox_primal(const float* primal, float tau, float lambda, const float* rho, const float* g, int64_t N,
+                    float* out, void* stream);
+
+/* TVL1OF3D_proxDual(dual [N,3,3], sigma, weight_tv) -> [N,3,3].
+ * Replaces opticalFlowOps.h:116-123 (opticalFlowOps.cu:475). */
+int svs_prox_dual(const float* dual, float sigma, float weight_tv, int64_t N, float* out, void* stream);
+
+/* ------------------------------------------------------------ warping */
+
+/* Warping3D.forward (C = 1, u [Z,Y,X]) and .forwardVectorField (u [Z,Y,X,C]):
+ * out(i) = u(i*h + phi(i)), phi [Z,Y,X,3].  Replaces warpingOps.h:92-110
+ * (cuda_warp3d, warpingOps.cu:1086; cuda_warpVectorField3d, :1265). */
+int svs_warp3d(const float* u, int C, const float* phi, int NZ, int NY, int NX, float hZ, float hY, float hX,
+               int interpolation, int bdry, float* out, void* stream);
+
+/* WarpingCNN3D.forward: u [B,C,Z,Y,X], phi [B,Z,Y,X,3] -> [B,C,Z,Y,X].
+ * Replaces warpingOpsCNN.h:21-34 (warpingOpsCNN.cu:133). */
+int svs_warp3d_cnn(const float* u, const float* phi, int B, int C, int NZ, int NY, int NX, float hZ, float hY,
+                   float hX, int interpolation, int bdry, float* out, void* stream);
+
+/* Adjoint of the warp with respect to the warped volume and the flow.
+ * B = C = 1 is Warping3D.backward(u, phi, grad_out) -> [grad_u, grad_phi]
+ * (warpingOps.h:98-103, warpingOps.cu:1173); B, C > 1 is the batched CNN form
+ * (grad_phi summed over channels), which the reference leaves commented out
+ * (warpingOpsCNN.h:36-41).  grad_u accumulates with float atomics, so its
+ * low bits depend on the order of the 64 scatters per voxel. */
+int svs_warp3d_backward(const float* u, const float* phi, const float* grad_out, int B, int C, int NZ, int NY,
+                        int NX, float hZ, float hY, float hX, int interpolation, int bdry, float* grad_u,
+                        float* grad_phi, void* stream);
+
+/* -------------------------------------------------------- resampling */
+
+/* Prolongation3D.forward / .forwardVectorField / .forwardMatrixField with C =
+ * 1 / 3 / 9 values per voxel, from mesh (N, L) to mesh (Np, Lp).  Replaces
+ * prolongationOps.h:86-103 (prolongationOps.cu:994, :1094, :1198). */
+int svs_prolongate3d(const float* u, int C, int NZ, int NY, int NX, float LZ, float LY, float LX, int NZp, int NYp,
+                     int NXp, float LZp, float LYp, float LXp, int interpolation, int bdry, float* out,
+                     void* stream);
+
+/* apply_gaussian_blur3d: zero-padded K^3 correlation with device weights
+ * [K,K,K].  Replaces the F.conv3d call of svs/utils/flow_utils.py:94-112. */
+int svs_blur3d(const float* in, int NZ, int NY, int NX, const float* weights, int K, float* out, void* stream);
+
+/* ---------------------------------------------------- the whole solve */
+
+/* Constructor arguments of TVL13DOpticalFlow (svs/modules/flow/tvl13d.py:14-67)
+ * plus launch-shape overrides.  Replaces TVL13DOpticalFlow.compute
+ * (tvl13d.py:122-255) and the ~27 launches + 3 host syncs per iteration it
+ * makes through the operator API. */
+typedef struct svs_tvl1_params {
+  int num_scales;            /* pyramid levels, sizes ceil(N/2) per level */
+  int max_warps;             /* warps per level */
+  int max_outer_iterations;  /* primal-dual iterations per warp */
+  float weight_matching;     /* lambda */
+  float weight_tv;
+  int pd_type;               /* 2: theta = 1/sqrt(1+2 gamma tau), tau *= theta, sigma /= theta */
+  double sigma, tau, theta, gamma;
+  int use_aniso;
+  float alpha, beta;
+  int use_blur;
+  int blur_K;                 /* odd kernel size */
+  const float* blur_weights;  /* host pointer, K^3 FP32 weights (flow_utils.py:74-91, 108-109) */
+  int interpolation, boundary;
+  int length_fixed;           /* 0: numDofs (L = N - 1); 1: the fixed lengths below */
+  float LZ, LY, LX;
+  int use_initial_state;      /* num_scales == 1 only: start from the bound u_in, p_in */
+  int sweep_bx, sweep_by, sweep_cz; /* fused-sweep work-group shape; 0 = automatic */
+} svs_tvl1_params;
+
+/* Opaque solve plan: pyramid geometry, workspace layout, device tables and an
+ * optional captured hipGraph for a fixed list of instance sizes. */
+typedef struct svs_plan svs_plan;
+
+/* Bytes of device workspace a plan for `ninst` instances needs; dims holds
+ * (NZ, NY, NX) per instance.  0 on invalid arguments. */
+size_t svs_tvl1_workspace_bytes(const svs_tvl1_params* params, int ninst, const int* dims);
+
+/* Builds a plan inside `workspace` (>= svs_tvl1_workspace_bytes).  Uploads the
+ * static tables on `stream` and synchronises it once. */
+int svs_tvl1_plan_create(const svs_tvl1_params* params, int ninst, const int* dims, void* workspace,
+                         size_t ws_bytes, void* stream, svs_plan** plan);
+
+/* Binds instance k to its frames I0, I1 [Z,Y,X], optional initial state
+ * u_in [Z,Y,X,3] / p_in [Z,Y,X,3,3] (read only when num_scales == 1 and
+ * use_initial_state) and outputs u_out [Z,Y,X,3] / p_out [Z,Y,X,3,3]
+ * (either may be NULL).  Takes effect at the next run. */
+int svs_tvl1_plan_bind(svs_plan* plan, int k, const float* I0, const float* I1, const float* u_in,
+                       const float* p_in, float* u_out, float* p_out);
+
+/* Enqueues every instance's solve on `stream`.  use_graph = 1 captures the
+ * fixed level x warp x iteration schedule as a hipGraph on first use and
+ * replays it; re-binding instances keeps the graph.  If bindings changed
+ * since the last run, the descriptor tables are re-uploaded and `stream` is
+ * synchronised once. */
+int svs_tvl1_plan_run(svs_plan* plan, int use_graph, void* stream);
+
+/* Frees the host side of a plan (the workspace belongs to the caller). */
+int svs_tvl1_plan_destroy(svs_plan* plan);
+
+/* Launch shape of one level: out[0..7] = bx, by, cz, tiles, blocks, ninst, levels, 0. */
+int svs_tvl1_plan_info(const svs_plan* plan, int level, int* out);
+
+/* Enqueues `count` fused primal-dual sweeps of pyramid level `level` on the
+ * state left by a previous run (measurement aid: times the sweep kernel on
+ * its own with events around it).  No reference counterpart. */
+int svs_tvl1_plan_sweeps(svs_plan* plan, int level, int count, void* stream);
+
+/* One frame pair without a graph: TVL13DOpticalFlow.compute(I0, I1, u, p). */
+int svs_tvl1_solve(const svs_tvl1_params* params, const float* I0, const float* I1, int NZ, int NY, int NX,
+                   const float* u_in, const float* p_in, float* u_out, float* p_out, void* workspace,
+                   size_t ws_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVSFLOW_H */

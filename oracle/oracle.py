@@ -1,0 +1,230 @@
+"""numpy front end of the CPU oracle (libsvs_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
+checker, and by bench.py's cpu_baseline leg.  Nothing in the product package
+imports this module.
+
+Every function takes and returns C-contiguous numpy arrays in the reference's
+layouts ([Z,Y,X], [Z,Y,X,3], [Z,Y,X,3,3], [B,C,Z,Y,X]).  ``dtype`` selects the
+literal FP32 restatement (np.float32) or the FP64 twin (np.float64).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libsvs_oracle.so")
+_lib = None
+
+BOUNDARY = {"NEAREST": 0, "MIRROR": 1, "REFLECT": 2}
+INTERP = {"NEAREST": 0, "LINEAR": 1, "CUBIC_HERMITESPLINE": 2}
+
+
+def build():
+    """Compile the oracle with its Makefile (g++, no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+    return _lib
+
+
+def _sfx(dtype):
+    return "f32" if np.dtype(dtype) == np.float32 else "f64"
+
+
+def _ct(dtype):
+    return ctypes.c_float if np.dtype(dtype) == np.float32 else ctypes.c_double
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _arr(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _fn(name, dtype):
+    return getattr(lib(), f"orc_{name}_{_sfx(dtype)}")
+
+
+def mesh_h(N, L, dtype=np.float32):
+    """h = L / (N - 1) as MeshInfo3D computes it (coreDefines.h:72-75)."""
+    if np.dtype(dtype) == np.float32:
+        return float(np.float32(L) / np.float32(N - 1))
+    return float(L) / float(N - 1)
+
+
+def bdry_index(i, N, bc):
+    return lib().orc_bdry_index_f32(int(i), int(N), int(bc))
+
+
+def cr_local(buf, lc, dtype=np.float32):
+    f = _fn("cr_local", dtype)
+    f.restype = _ct(dtype)
+    b = _arr(buf, dtype)
+    return f(_p(b), _ct(dtype)(lc))
+
+
+def _hs(h, dtype):
+    ct = _ct(dtype)
+    return [ct(h[0]), ct(h[1]), ct(h[2])]
+
+
+def warp(u, phi, h=(1.0, 1.0, 1.0), ip=2, bc=1, dtype=np.float32):
+    """Warping3D.forward (u [Z,Y,X]) or .forwardVectorField (u [Z,Y,X,C])."""
+    u = _arr(u, dtype)
+    phi = _arr(phi, dtype)
+    NZ, NY, NX = u.shape[:3]
+    C = 1 if u.ndim == 3 else u.shape[3]
+    out = np.empty_like(u)
+    _fn("warp", dtype)(_p(u), C, _p(phi), NZ, NY, NX, *_hs(h, dtype), ip, bc, _p(out))
+    return out
+
+
+def warp_cnn(u, phi, h=(1.0, 1.0, 1.0), ip=2, bc=1, dtype=np.float32):
+    u = _arr(u, dtype)
+    phi = _arr(phi, dtype)
+    B, C, NZ, NY, NX = u.shape
+    out = np.empty_like(u)
+    _fn("warp_cnn", dtype)(_p(u), _p(phi), B, C, NZ, NY, NX, *_hs(h, dtype), ip, bc, _p(out))
+    return out
+
+
+def warp_backward(u, phi, g, h=(1.0, 1.0, 1.0), ip=2, bc=1, dtype=np.float32):
+    u, phi, g = _arr(u, dtype), _arr(phi, dtype), _arr(g, dtype)
+    NZ, NY, NX = u.shape
+    gu = np.empty_like(u)
+    gphi = np.empty_like(phi)
+    _fn("warp_backward", dtype)(_p(u), _p(phi), _p(g), NZ, NY, NX, *_hs(h, dtype), ip, bc, _p(gu), _p(gphi))
+    return gu, gphi
+
+
+def warp_cnn_backward(u, phi, g, h=(1.0, 1.0, 1.0), ip=2, bc=1, dtype=np.float32):
+    u, phi, g = _arr(u, dtype), _arr(phi, dtype), _arr(g, dtype)
+    B, C, NZ, NY, NX = u.shape
+    gu = np.empty_like(u)
+    gphi = np.empty_like(phi)
+    _fn("warp_cnn_backward", dtype)(_p(u), _p(phi), _p(g), B, C, NZ, NY, NX, *_hs(h, dtype), ip, bc,
+                                    _p(gu), _p(gphi))
+    return gu, gphi
+
+
+def prolongate(u, src_L, dst_shape, dst_L, ip=2, bc=1, dtype=np.float32):
+    """Prolongation3D.forward / forwardVectorField / forwardMatrixField.
+
+    ``u`` is [Z,Y,X], [Z,Y,X,3] or [Z,Y,X,3,3]; lengths are (LZ, LY, LX)."""
+    u = _arr(u, dtype)
+    NZ, NY, NX = u.shape[:3]
+    trailing = u.shape[3:]
+    C = int(np.prod(trailing)) if trailing else 1
+    out = np.empty(tuple(dst_shape) + trailing, dtype=dtype)
+    ct = _ct(dtype)
+    _fn("prolongate", dtype)(_p(u), C, NZ, NY, NX, ct(src_L[0]), ct(src_L[1]), ct(src_L[2]),
+                             int(dst_shape[0]), int(dst_shape[1]), int(dst_shape[2]),
+                             ct(dst_L[0]), ct(dst_L[1]), ct(dst_L[2]), ip, bc, _p(out))
+    return out
+
+
+def nabla_fwd(b, h=(1.0, 1.0, 1.0), bc=1, dtype=np.float32):
+    """Nabla3D_CD.forward ([Z,Y,X] -> [..,3]) or .forwardVectorField ([..,3] -> [..,3,3])."""
+    b = _arr(b, dtype)
+    NZ, NY, NX = b.shape[:3]
+    C = 1 if b.ndim == 3 else b.shape[3]
+    out = np.empty(b.shape + (3,), dtype=dtype)
+    _fn("nabla_fwd", dtype)(_p(b), C, NZ, NY, NX, *_hs(h, dtype), bc, _p(out))
+    return out
+
+
+def nabla_bwd(p, h=(1.0, 1.0, 1.0), bc=1, dtype=np.float32):
+    """Nabla3D_CD.backward ([..,3] -> [Z,Y,X]) or .backwardVectorField ([..,3,3] -> [..,3])."""
+    p = _arr(p, dtype)
+    NZ, NY, NX = p.shape[:3]
+    C = 1 if p.ndim == 4 else p.shape[3]
+    out = np.empty(p.shape[:-1], dtype=dtype)
+    _fn("nabla_bwd", dtype)(_p(p), C, NZ, NY, NX, *_hs(h, dtype), bc, _p(out))
+    return out
+
+
+def aniso_tangents(G, alpha, beta, dtype=np.float32):
+    G = _arr(G, dtype)
+    N = G.size // 3
+    sh = G.shape[:-1]
+    s = np.empty(sh, dtype=dtype)
+    n, t1, t2 = (np.empty_like(G) for _ in range(3))
+    ct = _ct(dtype)
+    _fn("aniso_tangents", dtype)(_p(G), ctypes.c_longlong(N), ct(alpha), ct(beta), _p(s), _p(n), _p(t1), _p(t2))
+    return s, n, t1, t2
+
+
+def aniso_apply(b, s, n, t1, t2, dtype=np.float32):
+    b, s, n, t1, t2 = (_arr(a, dtype) for a in (b, s, n, t1, t2))
+    out = np.empty_like(b)
+    _fn("aniso_apply", dtype)(_p(b), _p(s), _p(n), _p(t1), _p(t2), ctypes.c_longlong(s.size), _p(out))
+    return out
+
+
+def prox_primal(primal, tau, lam, rho, g, dtype=np.float32):
+    primal, rho, g = _arr(primal, dtype), _arr(rho, dtype), _arr(g, dtype)
+    out = np.empty_like(primal)
+    ct = _ct(dtype)
+    _fn("prox_primal", dtype)(_p(primal), ct(tau), ct(lam), _p(rho), _p(g), ctypes.c_longlong(rho.size), _p(out))
+    return out
+
+
+def prox_dual(dual, wtv, dtype=np.float32):
+    dual = _arr(dual, dtype)
+    out = np.empty_like(dual)
+    _fn("prox_dual", dtype)(_p(dual), _ct(dtype)(wtv), ctypes.c_longlong(dual.size // 9), _p(out))
+    return out
+
+
+def blur(vol, weights, dtype=np.float32):
+    vol = _arr(vol, dtype)
+    w = _arr(weights, dtype)
+    K = w.shape[0]
+    out = np.empty_like(vol)
+    _fn("blur", dtype)(_p(vol), *vol.shape, _p(w), K, _p(out))
+    return out
+
+
+def solve_params(num_scales=4, max_warps=10, max_outer_iterations=50, weight_matching=1.0, weight_tv=0.025,
+                 pd_type=2, pd_params=(0.5, 0.5, 0.25, 1.0), use_aniso=True, aniso_params=(5.0, 0.5),
+                 interpolation="CUBIC_HERMITESPLINE", boundary="MIRROR", length_type="numDofs",
+                 length=(1, 1, 1)):
+    """Packs the TVL13DOpticalFlow constructor arguments (tvl13d.py:14-67).
+
+    ``length`` is (lx, ly, lz) as in conf/flow.yaml; the packed order is LZ, LY, LX."""
+    lx, ly, lz = length
+    return np.array([num_scales, max_warps, max_outer_iterations, weight_matching, weight_tv, pd_type,
+                     *pd_params, 1.0 if use_aniso else 0.0, *aniso_params, INTERP[interpolation],
+                     BOUNDARY[boundary], 1.0 if length_type == "fixed" else 0.0, lz, ly, lx], dtype=np.float64)
+
+
+def step_schedule(params):
+    n = int(params[2])
+    sig = np.empty(n, np.float64)
+    tau = np.empty(n, np.float64)
+    lib().orc_step_schedule(_p(params), _p(sig), _p(tau))
+    return sig, tau
+
+
+def solve(I0, I1, params, u=None, p=None, dtype=np.float32):
+    """TVL13DOpticalFlow.compute on already-blurred I0, I1 (tvl13d.py:122-162)."""
+    I0, I1 = _arr(I0, dtype), _arr(I1, dtype)
+    NZ, NY, NX = I0.shape
+    u = np.zeros((NZ, NY, NX, 3), dtype) if u is None else _arr(u, dtype)
+    p = np.zeros((NZ, NY, NX, 3, 3), dtype) if p is None else _arr(p, dtype)
+    uo = np.empty_like(u)
+    po = np.empty_like(p)
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    _fn("solve", dtype)(_p(params), _p(I0), _p(I1), NZ, NY, NX, _p(u), _p(p), _p(uo), _p(po))
+    return uo, po

@@ -1,0 +1,102 @@
+"""Loads libsvsflow.so (the HIP kernels behind a C ABI) through ctypes.
+
+There is no fallback: if the library is missing or cannot be loaded, every
+operator raises.  torch is imported first so that the library binds to the HIP
+runtime torch already loaded (both carry the soname libamdhip64.so.7).
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede loading the HIP library)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsvsflow.so")
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_i64 = ctypes.c_int64
+
+
+class TVL1Params(ctypes.Structure):
+    """Mirror of svs_tvl1_params (include/svsflow.h)."""
+
+    _fields_ = [
+        ("num_scales", _i), ("max_warps", _i), ("max_outer_iterations", _i),
+        ("weight_matching", _f), ("weight_tv", _f),
+        ("pd_type", _i),
+        ("sigma", ctypes.c_double), ("tau", ctypes.c_double), ("theta", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("use_aniso", _i), ("alpha", _f), ("beta", _f),
+        ("use_blur", _i), ("blur_K", _i), ("blur_weights", _vp),
+        ("interpolation", _i), ("boundary", _i),
+        ("length_fixed", _i), ("LZ", _f), ("LY", _f), ("LX", _f),
+        ("use_initial_state", _i),
+        ("sweep_bx", _i), ("sweep_by", _i), ("sweep_cz", _i),
+    ]
+
+
+_SIGNATURES = {
+    "svs_last_error": ([], ctypes.c_char_p),
+    "svs_fill": ([_vp, _i64, _f, _vp], _i),
+    "svs_nabla3d_forward": ([_vp, _i, _i, _i, _i, _f, _f, _f, _i, _vp, _vp], _i),
+    "svs_nabla3d_backward": ([_vp, _i, _i, _i, _i, _f, _f, _f, _i, _vp, _vp], _i),
+    "svs_aniso_tangents": ([_vp, _i64, _f, _f, _vp, _vp, _vp, _vp, _vp], _i),
+    "svs_aniso_apply": ([_vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp], _i),
+    "svs_prox_primal": ([_vp, _f, _f, _vp, _vp, _i64, _vp, _vp], _i),
+    "svs_prox_dual": ([_vp, _f, _f, _i64, _vp, _vp], _i),
+    "svs_warp3d": ([_vp, _i, _vp, _i, _i, _i, _f, _f, _f, _i, _i, _vp, _vp], _i),
+    "svs_warp3d_cnn": ([_vp, _vp, _i, _i, _i, _i, _i, _f, _f, _f, _i, _i, _vp, _vp], _i),
+    "svs_warp3d_backward": ([_vp, _vp, _vp, _i, _i, _i, _i, _i, _f, _f, _f, _i, _i, _vp, _vp, _vp], _i),
+    "svs_prolongate3d": ([_vp, _i, _i, _i, _i, _f, _f, _f, _i, _i, _i, _f, _f, _f, _i, _i, _vp, _vp], _i),
+    "svs_blur3d": ([_vp, _i, _i, _i, _vp, _i, _vp, _vp], _i),
+    "svs_tvl1_workspace_bytes": ([ctypes.POINTER(TVL1Params), _i, ctypes.POINTER(_i)], ctypes.c_size_t),
+    "svs_tvl1_plan_create": ([ctypes.POINTER(TVL1Params), _i, ctypes.POINTER(_i), _vp, ctypes.c_size_t, _vp,
+                              ctypes.POINTER(_vp)], _i),
+    "svs_tvl1_plan_bind": ([_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "svs_tvl1_plan_run": ([_vp, _i, _vp], _i),
+    "svs_tvl1_plan_destroy": ([_vp], _i),
+    "svs_tvl1_plan_info": ([_vp, _i, ctypes.POINTER(_i)], _i),
+    "svs_tvl1_plan_sweeps": ([_vp, _i, _i, _vp], _i),
+    "svs_tvl1_solve": ([ctypes.POINTER(TVL1Params), _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t,
+                        _vp], _i),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if libsvsflow.so was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m singleventriclesegmentation_amd.build` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(l, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = l
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def check(rc):
+    """Raises RuntimeError with the library's message for a non-zero status."""
+    if rc != 0:
+        msg = lib().svs_last_error().decode("utf-8", "replace")
+        raise RuntimeError(msg or f"libsvsflow call failed with status {rc}")
+
+
+def stream_ptr(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)

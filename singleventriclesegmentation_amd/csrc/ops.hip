@@ -1,0 +1,517 @@
+// Drop-in grid operators behind the opticalFlow operator API: central
+// differences and their adjoints, the anisotropic tensor, the two proxes,
+// warping (scalar, vector field, batched CNN form and the adjoint),
+// resampling between meshes and the Gaussian pre-blur.
+//
+// One thread per output voxel; x is the fastest index, so consecutive lanes
+// of a 64-wide wavefront touch consecutive addresses.  Every kernel writes
+// every element of its output, so callers may hand in uninitialised memory.
+#include "svs_common.hpp"
+#include "svs_device.hpp"
+#include "svs_math.h"
+
+#include <string.h>
+
+namespace svs {
+
+thread_local char g_last_error[1024] = "";
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------------ fill
+__global__ __launch_bounds__(kBlock) void k_fill(float* __restrict__ p, int64_t n, float v) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------- central differences
+// Nabla3D_CD.forward / forwardVectorField (differentialOps.cu:540-656,
+// :782-907): b [Z,Y,X,C] -> out [Z,Y,X,C,3].
+__global__ __launch_bounds__(kBlock) void k_nabla_fwd(const float* __restrict__ b, int C, int NZ, int NY, int NX, float hZ, float hY,
+                            float hX, int bc, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)NZ * NY * NX) return;
+  int z, y, x;
+  unflatten(i, NY, NX, z, y, x);
+  const int64_t sx = C, sy = (int64_t)NX * C, sz = (int64_t)NY * NX * C;
+  for (int c = 0; c < C; ++c) {
+    const float* p = b + i * C + c;
+    const float b0 = p[0];
+    const float xm = x > 0 ? p[-sx] : 0.0f, xp = x < NX - 1 ? p[sx] : 0.0f;
+    const float ym = y > 0 ? p[-sy] : 0.0f, yp = y < NY - 1 ? p[sy] : 0.0f;
+    const float zm = z > 0 ? p[-sz] : 0.0f, zp = z < NZ - 1 ? p[sz] : 0.0f;
+    float* o = out + (i * C + c) * 3;
+    o[0] = cd_fwd(xm, b0, xp, x, NX, hX, bc);
+    o[1] = cd_fwd(ym, b0, yp, y, NY, hY, bc);
+    o[2] = cd_fwd(zm, b0, zp, z, NZ, hZ, bc);
+  }
+}
+
+// Nabla3D_CD.backward / backwardVectorField (differentialOps.cu:686-772,
+// :909-1044): p [Z,Y,X,C,3] -> out [Z,Y,X,C] = ax/hX + ay/hY + az/hZ.
+__global__ __launch_bounds__(kBlock) void k_nabla_bwd(const float* __restrict__ p, int C, int NZ, int NY, int NX, float hZ, float hY,
+                            float hX, int bc, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)NZ * NY * NX) return;
+  int z, y, x;
+  unflatten(i, NY, NX, z, y, x);
+  const int64_t sx = 3 * C, sy = (int64_t)NX * 3 * C, sz = (int64_t)NY * NX * 3 * C;
+  for (int c = 0; c < C; ++c) {
+    const float* q = p + (i * C + c) * 3;
+    const float ax = cd_adj(x > 0 ? q[-sx] : 0.0f, q[0], x < NX - 1 ? q[sx] : 0.0f, x, NX, bc);
+    const float ay = cd_adj(y > 0 ? q[1 - sy] : 0.0f, q[1], y < NY - 1 ? q[1 + sy] : 0.0f, y, NY, bc);
+    const float az = cd_adj(z > 0 ? q[2 - sz] : 0.0f, q[2], z < NZ - 1 ? q[2 + sz] : 0.0f, z, NZ, bc);
+    out[i * C + c] = ax / hX + ay / hY + az / hZ;
+  }
+}
+
+// ------------------------------------------------ anisotropic tensor
+// AnisotropicNabla3D.computeTangentVecs (anisotropicDifferentialOps.cu:90-141).
+__global__ __launch_bounds__(kBlock) void k_aniso_tangents(const float* __restrict__ G, int64_t N, float alpha, float beta,
+                                 float* __restrict__ s, float* __restrict__ n, float* __restrict__ t1,
+                                 float* __restrict__ t2) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  const float g[3] = {G[3 * i], G[3 * i + 1], G[3 * i + 2]};
+  float ns = 0.0f;
+  for (int d = 0; d < 3; ++d) ns += g[d] * g[d];
+  const float nrm = sqrtf(ns);
+  if (nrm > 0.0f) {
+    s[i] = svs_expf(-alpha * svs_powf(nrm, beta));
+    float nn[3], tt[3];
+    for (int d = 0; d < 3; ++d) nn[d] = g[d] / nrm;
+    tangent1(nn, tt);
+    for (int d = 0; d < 3; ++d) { n[3 * i + d] = nn[d]; t1[3 * i + d] = tt[d]; }
+    t2[3 * i + 0] = nn[1] * tt[2] - nn[2] * tt[1];
+    t2[3 * i + 1] = nn[2] * tt[0] - nn[0] * tt[2];
+    t2[3 * i + 2] = nn[0] * tt[1] - nn[1] * tt[0];
+  } else {
+    s[i] = 0.0f;
+    for (int d = 0; d < 3; ++d) { n[3 * i + d] = 0.0f; t1[3 * i + d] = 0.0f; t2[3 * i + d] = 0.0f; }
+  }
+}
+
+// AnisotropicNabla3D.forwardVectorField / backwardVectorField
+// (anisotropicDifferentialOps.cu:144-175); the t2 factor is a literal 0.
+__global__ __launch_bounds__(kBlock) void k_aniso_apply(const float* __restrict__ in, const float* __restrict__ s,
+                              const float* __restrict__ n, const float* __restrict__ t1,
+                              const float* __restrict__ t2, int64_t N, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  const float nn[3] = {n[3 * i], n[3 * i + 1], n[3 * i + 2]};
+  const float tt[3] = {t1[3 * i], t1[3 * i + 1], t1[3 * i + 2]};
+  const float t2v[3] = {t2[3 * i], t2[3 * i + 1], t2[3 * i + 2]};
+  const float si = s[i];
+  const float t2f = 0.0f;
+  for (int c = 0; c < 3; ++c) {
+    const float q[3] = {in[9 * i + 3 * c], in[9 * i + 3 * c + 1], in[9 * i + 3 * c + 2]};
+    float o[3];
+    aniso_row(si, nn, tt, q, o);
+    for (int d = 0; d < 3; ++d) out[9 * i + 3 * c + d] = o[d] + t2f * t2v[d];
+  }
+}
+
+// ------------------------------------------------------------ proxes
+// TVL1OF3D_proxPrimal (opticalFlowOps.cu:123-175).
+__global__ __launch_bounds__(kBlock) void k_prox_primal(const float* __restrict__ primal, float factor, const float* __restrict__ rho,
+                              const float* __restrict__ g, int64_t N, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  float v[3] = {primal[3 * i], primal[3 * i + 1], primal[3 * i + 2]};
+  const float gg[3] = {g[3 * i], g[3 * i + 1], g[3 * i + 2]};
+  prox_primal_point(v, rho[i], gg, factor);
+  for (int c = 0; c < 3; ++c) out[3 * i + c] = v[c];
+}
+
+// TVL1OF3D_proxDual (opticalFlowOps.cu:179-208).
+__global__ __launch_bounds__(kBlock) void k_prox_dual(const float* __restrict__ dual, float wtv, int64_t N, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  for (int c = 0; c < 3; ++c) {
+    float v[3] = {dual[9 * i + 3 * c], dual[9 * i + 3 * c + 1], dual[9 * i + 3 * c + 2]};
+    prox_dual_row(v, wtv);
+    for (int d = 0; d < 3; ++d) out[9 * i + 3 * c + d] = v[d];
+  }
+}
+
+// ------------------------------------------------------------- warping
+// One strided scalar component of a [Z,Y,X,C] (or [B,C,Z,Y,X]) field.
+struct StridedFetch {
+  const float* base;
+  int64_t stride;
+  int NY, NX;
+  __device__ void operator()(int z, int y, int x, float* v) const {
+    v[0] = base[(((int64_t)z * NY + y) * NX + x) * stride];
+  }
+};
+
+// Warping3D.forward / forwardVectorField (warpingOps.cu:572-596, :632-660):
+// out(i) = u(i*h + phi(i)), u [Z,Y,X,C].
+__global__ __launch_bounds__(kBlock) void k_warp(const float* __restrict__ u, int C, const float* __restrict__ phi, int NZ, int NY,
+                       int NX, float hZ, float hY, float hX, int ip, int bc, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)NZ * NY * NX) return;
+  int z, y, x;
+  unflatten(i, NY, NX, z, y, x);
+  const float cx = (float)x * hX + phi[3 * i + 0];
+  const float cy = (float)y * hY + phi[3 * i + 1];
+  const float cz = (float)z * hZ + phi[3 * i + 2];
+  for (int c = 0; c < C; ++c) {
+    StridedFetch f{u + c, C, NY, NX};
+    float v[1];
+    sample<1>(ip, f, NZ, NY, NX, hZ, hY, hX, bc, cz, cy, cx, v);
+    out[i * C + c] = v[0];
+  }
+}
+
+// WarpingCNN3D.forward (warpingOpsCNN.cu:97-128): u [B,C,Z,Y,X], phi [B,Z,Y,X,3].
+__global__ __launch_bounds__(kBlock) void k_warp_cnn(const float* __restrict__ u, const float* __restrict__ phi, int B, int C, int NZ,
+                           int NY, int NX, float hZ, float hY, float hX, int ip, int bc, float* __restrict__ out) {
+  const int64_t N = (int64_t)NZ * NY * NX;
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)B * N) return;
+  const int b = (int)(t / N);
+  const int64_t i = t - (int64_t)b * N;
+  int z, y, x;
+  unflatten(i, NY, NX, z, y, x);
+  const float* ph = phi + ((int64_t)b * N + i) * 3;
+  const float cx = (float)x * hX + ph[0];
+  const float cy = (float)y * hY + ph[1];
+  const float cz = (float)z * hZ + ph[2];
+  for (int c = 0; c < C; ++c) {
+    const int64_t off = ((int64_t)b * C + c) * N;
+    StridedFetch f{u + off, 1, NY, NX};
+    float v[1];
+    sample<1>(ip, f, NZ, NY, NX, hZ, hY, hX, bc, cz, cy, cx, v);
+    out[off + i] = v[0];
+  }
+}
+
+// Adjoint of one tricubic sample (interpolation_cubicHermiteSpline.cuh:527-636):
+// scatters fwd * d out/d u into grad (atomics, 64 taps) and adds the three
+// flow derivatives to gphi in the reference's order (z, y per dz, x per dy).
+__device__ void warp_backward_cubic(const float* __restrict__ u, float* __restrict__ grad, int NZ, int NY, int NX,
+                                    float hZ, float hY, float hX, int bc, float cz, float cy, float cx, float fwd,
+                                    float gphi[3]) {
+  const Axis ax = make_axis(cx, hX), ay = make_axis(cy, hY), az = make_axis(cz, hZ);
+  const float lx = ax.w + 1, ly = ay.w + 1, lz = az.w + 1;
+  int xs[4], ys[4], zs[4];
+  for (int k = 0; k < 4; ++k) {
+    xs[k] = bidx(ax.f - 1 + k, NX, bc);
+    ys[k] = bidx(ay.f - 1 + k, NY, bc);
+    zs[k] = bidx(az.f - 1 + k, NZ, bc);
+  }
+  auto U = [&](int z, int y, int x) { return u[((int64_t)z * NY + y) * NX + x]; };
+  float bz[4], by[4];
+  float byz[4][4];  // x-interpolated values per (dz, dy)
+  for (int dz = 0; dz < 4; ++dz) {
+    for (int dy = 0; dy < 4; ++dy) {
+      const int z = zs[dz], y = ys[dy];
+      byz[dz][dy] = cr_local(U(z, y, xs[0]), U(z, y, xs[1]), U(z, y, xs[2]), U(z, y, xs[3]), lx);
+    }
+    bz[dz] = cr_local(byz[dz][0], byz[dz][1], byz[dz][2], byz[dz][3], ly);
+  }
+  float gz[4];
+  const float gwz = cr_local_backward(bz[0], bz[1], bz[2], bz[3], lz, fwd, gz);
+  gphi[2] += gwz / hZ;
+  for (int dz = 0; dz < 4; ++dz) {
+    for (int dy = 0; dy < 4; ++dy) by[dy] = byz[dz][dy];
+    float gy[4];
+    const float gwy = cr_local_backward(by[0], by[1], by[2], by[3], ly, gz[dz], gy);
+    gphi[1] += gwy / hY;
+    for (int dy = 0; dy < 4; ++dy) {
+      const int z = zs[dz], y = ys[dy];
+      float gx[4];
+      const float gwx = cr_local_backward(U(z, y, xs[0]), U(z, y, xs[1]), U(z, y, xs[2]), U(z, y, xs[3]), lx,
+                                          gy[dy], gx);
+      gphi[0] += gwx / hX;
+      for (int dx = 0; dx < 4; ++dx) unsafeAtomicAdd(grad + ((int64_t)z * NY + y) * NX + xs[dx], gx[dx]);
+    }
+  }
+}
+
+// Adjoint of one trilinear sample (interpolation_linear.cuh:297-363).
+__device__ void warp_backward_linear(const float* __restrict__ u, float* __restrict__ grad, int NZ, int NY, int NX,
+                                     float hZ, float hY, float hX, int bc, float cz, float cy, float cx, float fv,
+                                     float gphi[3]) {
+  const Axis ax = make_axis(cx, hX), ay = make_axis(cy, hY), az = make_axis(cz, hZ);
+  const float wx = ax.w, wy = ay.w, wz = az.w;
+  const int xf = bidx(ax.f, NX, bc), xc = bidx(ax.f + 1, NX, bc);
+  const int yf = bidx(ay.f, NY, bc), yc = bidx(ay.f + 1, NY, bc);
+  const int zf = bidx(az.f, NZ, bc), zc = bidx(az.f + 1, NZ, bc);
+  auto I = [&](int z, int y, int x) { return ((int64_t)z * NY + y) * NX + x; };
+  const float u_fff = u[I(zf, yf, xf)], u_ffc = u[I(zc, yf, xf)], u_fcf = u[I(zf, yc, xf)], u_fcc = u[I(zc, yc, xf)];
+  const float u_cff = u[I(zf, yf, xc)], u_cfc = u[I(zc, yf, xc)], u_ccf = u[I(zf, yc, xc)], u_ccc = u[I(zc, yc, xc)];
+  unsafeAtomicAdd(grad + I(zf, yf, xf), (1 - wz) * (1 - wy) * (1 - wx) * fv);
+  unsafeAtomicAdd(grad + I(zf, yc, xf), (1 - wz) * wy * (1 - wx) * fv);
+  unsafeAtomicAdd(grad + I(zf, yf, xc), (1 - wz) * (1 - wy) * wx * fv);
+  unsafeAtomicAdd(grad + I(zf, yc, xc), (1 - wz) * wy * wx * fv);
+  unsafeAtomicAdd(grad + I(zc, yf, xf), wz * (1 - wy) * (1 - wx) * fv);
+  unsafeAtomicAdd(grad + I(zc, yc, xf), wz * wy * (1 - wx) * fv);
+  unsafeAtomicAdd(grad + I(zc, yf, xc), wz * (1 - wy) * wx * fv);
+  unsafeAtomicAdd(grad + I(zc, yc, xc), wz * wy * wx * fv);
+  gphi[0] += ((1 - wz) * (1 - wy) * (u_cff - u_fff) + (1 - wz) * wy * (u_ccf - u_fcf) +
+              wz * (1 - wy) * (u_cfc - u_ffc) + wz * wy * (u_ccc - u_fcc)) / hX * fv;
+  gphi[1] += ((1 - wz) * (1 - wx) * (u_fcf - u_fff) + (1 - wz) * wx * (u_ccf - u_cff) +
+              wz * (1 - wx) * (u_fcc - u_ffc) + wz * wx * (u_ccc - u_cfc)) / hY * fv;
+  gphi[2] += ((1 - wy) * (1 - wx) * (u_ffc - u_fff) + (1 - wy) * wx * (u_cfc - u_cff) +
+              wy * (1 - wx) * (u_fcc - u_fcf) + wy * wx * (u_ccc - u_ccf)) / hZ * fv;
+}
+
+// Warping3D.backward (warpingOps.cu:598-630, :1173-1262) generalised to the
+// batched CNN layout: u/grad_u [B,C,Z,Y,X], phi/grad_phi [B,Z,Y,X,3].  The
+// flow gradient sums the per-channel adjoints in channel order.  grad_u must
+// be zero on entry (the launcher clears it).
+__global__ __launch_bounds__(kBlock) void k_warp_backward(const float* __restrict__ u, const float* __restrict__ phi,
+                                const float* __restrict__ gout, int B, int C, int NZ, int NY, int NX, float hZ,
+                                float hY, float hX, int ip, int bc, float* __restrict__ grad_u,
+                                float* __restrict__ grad_phi) {
+  const int64_t N = (int64_t)NZ * NY * NX;
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)B * N) return;
+  const int b = (int)(t / N);
+  const int64_t i = t - (int64_t)b * N;
+  int z, y, x;
+  unflatten(i, NY, NX, z, y, x);
+  const float* ph = phi + ((int64_t)b * N + i) * 3;
+  const float cx = (float)x * hX + ph[0];
+  const float cy = (float)y * hY + ph[1];
+  const float cz = (float)z * hZ + ph[2];
+  float gphi[3] = {0.0f, 0.0f, 0.0f};
+  for (int c = 0; c < C; ++c) {
+    const int64_t off = ((int64_t)b * C + c) * N;
+    const float fv = gout[off + i];
+    if (ip == IP_CUBIC) {
+      warp_backward_cubic(u + off, grad_u + off, NZ, NY, NX, hZ, hY, hX, bc, cz, cy, cx, fv, gphi);
+    } else if (ip == IP_LINEAR) {
+      warp_backward_linear(u + off, grad_u + off, NZ, NY, NX, hZ, hY, hX, bc, cz, cy, cx, fv, gphi);
+    } else {
+      const int xx = nearest_axis(cx, hX, NX, bc), yy = nearest_axis(cy, hY, NY, bc),
+                zz = nearest_axis(cz, hZ, NZ, bc);
+      unsafeAtomicAdd(grad_u + off + ((int64_t)zz * NY + yy) * NX + xx, fv);
+    }
+  }
+  float* gp = grad_phi + ((int64_t)b * N + i) * 3;
+  gp[0] = gphi[0];
+  gp[1] = gphi[1];
+  gp[2] = gphi[2];
+}
+
+// --------------------------------------------------------- prolongation
+// Prolongation3D.forward / forwardVectorField / forwardMatrixField
+// (prolongationOps.cu:533-625): source coordinate ((i * hP) * L) / LP.
+__global__ __launch_bounds__(kBlock) void k_prolongate(const float* __restrict__ u, int C, int NZ, int NY, int NX, float LZ, float LY,
+                             float LX, float hZ, float hY, float hX, int NZp, int NYp, int NXp, float LZp,
+                             float LYp, float LXp, float hZp, float hYp, float hXp, int ip, int bc,
+                             float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)NZp * NYp * NXp) return;
+  int z, y, x;
+  unflatten(i, NYp, NXp, z, y, x);
+  const float cxp = (float)x * hXp;
+  const float cx = cxp * LX / LXp;
+  const float cyp = (float)y * hYp;
+  const float cy = cyp * LY / LYp;
+  const float czp = (float)z * hZp;
+  const float cz = czp * LZ / LZp;
+  for (int c = 0; c < C; ++c) {
+    StridedFetch f{u + c, C, NY, NX};
+    float v[1];
+    sample<1>(ip, f, NZ, NY, NX, hZ, hY, hX, bc, cz, cy, cx, v);
+    out[i * C + c] = v[0];
+  }
+}
+
+// ----------------------------------------------------------------- blur
+// apply_gaussian_blur3d (flow_utils.py:94-112): zero-padded K^3 correlation,
+// taps summed in (z, y, x) order.
+__global__ __launch_bounds__(kBlock) void k_blur(const float* __restrict__ in, int NZ, int NY, int NX, const float* __restrict__ w, int K,
+                       float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)NZ * NY * NX) return;
+  int z, y, x;
+  unflatten(i, NY, NX, z, y, x);
+  const int P = K / 2;
+  float acc = 0.0f;
+  for (int a = 0; a < K; ++a) {
+    const int zz = z + a - P;
+    if (zz < 0 || zz >= NZ) continue;
+    for (int b = 0; b < K; ++b) {
+      const int yy = y + b - P;
+      if (yy < 0 || yy >= NY) continue;
+      const float* row = in + ((int64_t)zz * NY + yy) * NX;
+      const float* wr = w + (a * K + b) * K;
+      for (int c = 0; c < K; ++c) {
+        const int xx = x + c - P;
+        if (xx < 0 || xx >= NX) continue;
+        acc = acc + wr[c] * row[xx];
+      }
+    }
+  }
+  out[i] = acc;
+}
+
+int launch_fill(float* p, int64_t n, float v, hipStream_t s) {
+  if (n <= 0) return SVS_STATUS_OK;
+  k_fill<<<grid_for(n, kBlock), kBlock, 0, s>>>(p, n, v);
+  SVS_LAUNCHED();
+  return SVS_STATUS_OK;
+}
+
+}  // namespace svs
+
+// ================================================================ C ABI
+using namespace svs;
+
+extern "C" {
+
+const char* svs_last_error(void) { return g_last_error; }
+
+int svs_fill(float* out, int64_t n, float value, void* stream) {
+  SVS_REQUIRE(n >= 0 && (n == 0 || out), "svs_fill: bad arguments");
+  return launch_fill(out, n, value, (hipStream_t)stream);
+}
+
+int svs_nabla3d_forward(const float* b, int C, int NZ, int NY, int NX, float hZ, float hY, float hX, int bdry,
+                        float* out, void* stream) {
+  SVS_REQUIRE(b && out, "Nabla3D_CD.forward: null pointer");
+  SVS_REQUIRE(C >= 1, "Nabla3D_CD.forward: C must be >= 1");
+  SVS_REQUIRE_DIMS(NZ, NY, NX);
+  SVS_REQUIRE(valid_bc(bdry), "Nabla3D_CD.forward: invalid boundary type %d", bdry);
+  const int64_t n = (int64_t)NZ * NY * NX;
+  k_nabla_fwd<<<grid_for(n, kBlock), kBlock, 0, (hipStream_t)stream>>>(b, C, NZ, NY, NX, hZ, hY, hX, bdry, out);
+  SVS_LAUNCHED();
+  return SVS_STATUS_OK;
+}
+
+int svs_nabla3d_backward(const float* p, int C, int NZ, int NY, int NX, float hZ, float hY, float hX, int bdry,
+                         float* out, void* stream) {
+  SVS_REQUIRE(p && out, "Nabla3D_CD.backward: null pointer");
+  SVS_REQUIRE(C >= 1, "Nabla3D_CD.backward: C must be >= 1");
+  SVS_REQUIRE_DIMS(NZ, NY, NX);
+  SVS_REQUIRE(valid_bc(bdry), "Nabla3D_CD.backward: invalid boundary type %d", bdry);
+  const int64_t n = (int64_t)NZ * NY * NX;
+  k_nabla_bwd<<<grid_for(n, kBlock), kBlock, 0, (hipStream_t)stream>>>(p, C, NZ, NY, NX, hZ, hY, hX, bdry, out);
+  SVS_LAUNCHED();
+  return SVS_STATUS_OK;
+}
+
+int svs_aniso_tangents(const float* G, int64_t N, float alpha, float beta, float* s, float* n, float* t1, float* t2,
+                       void* stream) {
+  SVS_REQUIRE(G && s && n && t1 && t2, "AnisotropicNabla3D.computeTangentVecs: null pointer");
+  SVS_REQUIRE(N >= 0, "AnisotropicNabla3D.computeTangentVecs: negative size");
+  if (N == 0) return SVS_STATUS_OK;
+  k_aniso_tangents<<<grid_for(N, kBlock), kBlock, 0, (hipStream_t)stream>>>(G, N, alpha, beta, s, n, t1, t2);
+  SVS_LAUNCHED();
+  return SVS_STATUS_OK;
+}
+
+int svs_aniso_apply(const float* in, const float* s, const float* n, const float* t1, const float* t2, int64_t N,
+                    float* out, void* stream) {
+  SVS_REQUIRE(in && s && n && t1 && t2 && out, "AnisotropicNabla3D.forwardVectorField: null pointer");
+  SVS_REQUIRE(N >= 0, "AnisotropicNabla3D.forwardVectorField: negative size");
+  if (N == 0) return SVS_STATUS_OK;
+  k_aniso_apply<<<grid_for(N, kBlock), kBlock, 0, (hipStream_t)stream>>>(in, s, n, t1, t2, N, out);
+  SVS_LAUNCHED();
+  return SVS_STATUS_OK;
+}
+
+int svs_prox_primal(const float* primal, float tau, float lambda, const float* rho, const float* g, int64_t N,
+                    float* out, void* stream) {
+  SVS_REQUIRE(primal && rho && g && out, "TVL1OF3D_proxPrimal: null pointer");
+  SVS_REQUIRE(N >= 0, "TVL1OF3D_proxPrimal: negative size");
+  if (N == 0) return SVS_STATUS_OK;
+  const float factor = tau * lambda;  // opticalFlowOps.cu:442
+  k_prox_primal<<<grid_for(N, kBlock), kBlock, 0, (hipStream_t)stream>>>(primal, factor, rho, g, N, out);
+  SVS_LAUNCHED();
+  return SVS_STATUS_OK;
+}
+
+int svs_prox_dual(const float* dual, float sigma, float wtv, int64_t N, float* out, void* stream) {
+  (void)sigma;  // the reference passes sigma but its kernel never reads it (opticalFlowOps.cu:475-521)
+  SVS_REQUIRE(dual && out, "TVL1OF3D_proxDual: null pointer");
+  SVS_REQUIRE(N >= 0, "TVL1OF3D_proxDual: negative size");
+  if (N == 0) return SVS_STATUS_OK;
+  k_prox_dual<<<grid_for(N, kBlock), kBlock, 0, (hipStream_t)stream>>>(dual, wtv, N, out);
+  SVS_LAUNCHED();
+  return SVS_STATUS_OK;
+}
+
+int svs_warp3d(const float* u, int C, const float* phi, int NZ, int NY, int NX, float hZ, float hY, float hX,
+               int interp, int bdry, float* out, void* stream) {
+  SVS_REQUIRE(u && phi && out, "Warping3D.forward: null pointer");
+  SVS_REQUIRE(C >= 1, "Warping3D.forward: C must be >= 1");
+  SVS_REQUIRE_DIMS(NZ, NY, NX);
+  SVS_REQUIRE(valid_bc(bdry) && valid_ip(interp), "Warping3D.forward: invalid interpolation/boundary type");
+  const int64_t n = (int64_t)NZ * NY * NX;
+  k_warp<<<grid_for(n, kBlock), kBlock, 0, (hipStream_t)stream>>>(u, C, phi, NZ, NY, NX, hZ, hY, hX, interp, bdry,
+                                                                  out);
+  SVS_LAUNCHED();
+  return SVS_STATUS_OK;
+}
+
+int svs_warp3d_cnn(const float* u, const float* phi, int B, int C, int NZ, int NY, int NX, float hZ, float hY,
+                   float hX, int interp, int bdry, float* out, void* stream) {
+  SVS_REQUIRE(u && phi && out, "WarpingCNN3D.forward: null pointer");
+  SVS_REQUIRE(B >= 1 && C >= 1, "WarpingCNN3D.forward: batch and channels must be >= 1");
+  SVS_REQUIRE_DIMS(NZ, NY, NX);
+  SVS_REQUIRE(valid_bc(bdry) && valid_ip(interp), "WarpingCNN3D.forward: invalid interpolation/boundary type");
+  const int64_t n = (int64_t)B * NZ * NY * NX;
+  k_warp_cnn<<<grid_for(n, kBlock), kBlock, 0, (hipStream_t)stream>>>(u, phi, B, C, NZ, NY, NX, hZ, hY, hX, interp,
+                                                                      bdry, out);
+  SVS_LAUNCHED();
+  return SVS_STATUS_OK;
+}
+
+int svs_warp3d_backward(const float* u, const float* phi, const float* grad_out, int B, int C, int NZ, int NY,
+                        int NX, float hZ, float hY, float hX, int interp, int bdry, float* grad_u, float* grad_phi,
+                        void* stream) {
+  SVS_REQUIRE(u && phi && grad_out && grad_u && grad_phi, "Warping3D.backward: null pointer");
+  SVS_REQUIRE(B >= 1 && C >= 1, "Warping3D.backward: batch and channels must be >= 1");
+  SVS_REQUIRE_DIMS(NZ, NY, NX);
+  SVS_REQUIRE(valid_bc(bdry) && valid_ip(interp), "Warping3D.backward: invalid interpolation/boundary type");
+  const int64_t n = (int64_t)NZ * NY * NX;
+  int rc = launch_fill(grad_u, (int64_t)B * C * n, 0.0f, (hipStream_t)stream);
+  if (rc) return rc;
+  k_warp_backward<<<grid_for((int64_t)B * n, kBlock), kBlock, 0, (hipStream_t)stream>>>(
+      u, phi, grad_out, B, C, NZ, NY, NX, hZ, hY, hX, interp, bdry, grad_u, grad_phi);
+  SVS_LAUNCHED();
+  return SVS_STATUS_OK;
+}
+
+int svs_prolongate3d(const float* u, int C, int NZ, int NY, int NX, float LZ, float LY, float LX, int NZp, int NYp,
+                     int NXp, float LZp, float LYp, float LXp, int interp, int bdry, float* out, void* stream) {
+  SVS_REQUIRE(u && out, "Prolongation3D.forward: null pointer");
+  SVS_REQUIRE(C >= 1, "Prolongation3D.forward: C must be >= 1");
+  SVS_REQUIRE_DIMS(NZ, NY, NX);
+  SVS_REQUIRE_DIMS(NZp, NYp, NXp);
+  SVS_REQUIRE(valid_bc(bdry) && valid_ip(interp), "Prolongation3D.forward: invalid interpolation/boundary type");
+  const float hZ = LZ / (float)(NZ - 1), hY = LY / (float)(NY - 1), hX = LX / (float)(NX - 1);
+  const float hZp = LZp / (float)(NZp - 1), hYp = LYp / (float)(NYp - 1), hXp = LXp / (float)(NXp - 1);
+  const int64_t n = (int64_t)NZp * NYp * NXp;
+  k_prolongate<<<grid_for(n, kBlock), kBlock, 0, (hipStream_t)stream>>>(u, C, NZ, NY, NX, LZ, LY, LX, hZ, hY, hX,
+                                                                        NZp, NYp, NXp, LZp, LYp, LXp, hZp, hYp,
+                                                                        hXp, interp, bdry, out);
+  SVS_LAUNCHED();
+  return SVS_STATUS_OK;
+}
+
+int svs_blur3d(const float* in, int NZ, int NY, int NX, const float* weights, int K, float* out, void* stream) {
+  SVS_REQUIRE(in && weights && out, "apply_gaussian_blur3d: null pointer");
+  SVS_REQUIRE(K >= 1 && (K % 2) == 1, "apply_gaussian_blur3d: kernel size must be odd and positive (got %d)", K);
+  SVS_REQUIRE_DIMS(NZ, NY, NX);
+  const int64_t n = (int64_t)NZ * NY * NX;
+  k_blur<<<grid_for(n, kBlock), kBlock, 0, (hipStream_t)stream>>>(in, NZ, NY, NX, weights, K, out);
+  SVS_LAUNCHED();
+  return SVS_STATUS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,869 @@
+// The whole TV-L1 solve on the device: pyramid, per-warp linearisation and
+// the fused primal-dual sweep, for one frame pair or a batch of frame pairs
+// of uneven sizes (tvl13d.py:69-255 is the algorithm restated here).
+//
+// Every kernel reads its per-instance geometry and buffer addresses from a
+// device-side descriptor table (LevelDesc).  A captured hipGraph therefore
+// holds only table addresses and per-iteration scalars; binding a plan to new
+// inputs or outputs rewrites the table and keeps the graph.
+//
+// State layout in HBM, per instance and pyramid level (N voxels, x fastest):
+//   I0, I1     float[N]            restricted (level 0: blurred) frames
+//   J          float4[N]           (I1, dI1/dx, dI1/dy, dI1/dz)  gather source
+//   SN         float4[N]           (s, n_x, n_y, n_z) anisotropic weight+normal
+//   RG         float4[N]           (rho_c, g_x, g_y, g_z) from the warp
+//   u[2]       float[3][N]  SoA    flow, ping-pong
+//   p[2]       float[9][N]  SoA    dual, p[3c+d] = row c, derivative d
+// J, SN and RG are shared by the levels of an instance (levels run in turn).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "svs_common.hpp"
+#include "svs_device.hpp"
+#include "svs_math.h"
+#include "svsflow.h"
+
+namespace svs {
+
+int launch_fill(float* p, int64_t n, float v, hipStream_t s);
+
+struct alignas(16) LevelDesc {
+  int NZ, NY, NX, pad0;
+  float hZ, hY, hX, pad1;
+  float LZ, LY, LX, pad2;
+  float upZ, upY, upX, pad3;  // coarse -> this level flow scale L_fine / L_coarse
+  int64_t N;
+  float* I0;
+  float* I1;
+  float4* J;
+  float4* SN;
+  float4* RG;
+  float* u[2];
+  float* p[2];
+  // Level 0 only: caller-bound inputs and outputs.
+  const float* src0;
+  const float* src1;
+  const float* u_in;
+  const float* p_in;
+  float* u_out;
+  float* p_out;
+};
+
+constexpr int kBlock = 256;
+
+// Maps a block of a flattened multi-instance launch to (instance, voxel).
+__device__ __forceinline__ int find_instance(const int* __restrict__ block_start, int ninst, int blk) {
+  int lo = 0, hi = ninst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (block_start[mid] <= blk) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+#define SVS_BATCH_PROLOGUE(descs)                                                  \
+  const int inst_ = find_instance(block_start, ninst, (int)blockIdx.x);            \
+  const LevelDesc& d = descs[inst_];                                               \
+  const int64_t i = (int64_t)(blockIdx.x - block_start[inst_]) * kBlock + threadIdx.x; \
+  if (i >= d.N) return;                                                            \
+  int z, y, x;                                                                     \
+  unflatten(i, d.NY, d.NX, z, y, x)
+
+// ---------------------------------------------------------- level set-up
+// Gaussian pre-blur of both frames of every instance (flow_utils.py:94-112).
+__global__ __launch_bounds__(kBlock) void k_blur_pair(const LevelDesc* __restrict__ descs, const int* __restrict__ block_start, int ninst,
+                            const float* __restrict__ w, int K) {
+  SVS_BATCH_PROLOGUE(descs);
+  const int P = K / 2;
+  float acc0 = 0.0f, acc1 = 0.0f;
+  for (int a = 0; a < K; ++a) {
+    const int zz = z + a - P;
+    if (zz < 0 || zz >= d.NZ) continue;
+    for (int b = 0; b < K; ++b) {
+      const int yy = y + b - P;
+      if (yy < 0 || yy >= d.NY) continue;
+      const int64_t row = ((int64_t)zz * d.NY + yy) * d.NX;
+      const float* wr = w + (a * K + b) * K;
+      for (int c = 0; c < K; ++c) {
+        const int xx = x + c - P;
+        if (xx < 0 || xx >= d.NX) continue;
+        acc0 = acc0 + wr[c] * d.src0[row + xx];
+        acc1 = acc1 + wr[c] * d.src1[row + xx];
+      }
+    }
+  }
+  d.I0[i] = acc0;
+  d.I1[i] = acc1;
+}
+
+struct PlaneFetch {
+  const float* v;
+  int NY, NX;
+  __device__ void operator()(int z, int y, int x, float* o) const { o[0] = v[((int64_t)z * NY + y) * NX + x]; }
+};
+
+// Pyramid restriction fine -> coarse by resampling (tvl13d.py:108-115,
+// prolongationOps.cu:533-559); both frames share the sample geometry.
+__global__ __launch_bounds__(kBlock) void k_restrict(const LevelDesc* __restrict__ fine, const LevelDesc* __restrict__ coarse,
+                           const int* __restrict__ block_start, int ninst, int ip, int bc) {
+  SVS_BATCH_PROLOGUE(coarse);
+  const LevelDesc& f = fine[inst_];
+  const float cxp = (float)x * d.hX;
+  const float cx = cxp * f.LX / d.LX;
+  const float cyp = (float)y * d.hY;
+  const float cy = cyp * f.LY / d.LY;
+  const float czp = (float)z * d.hZ;
+  const float cz = czp * f.LZ / d.LZ;
+  float v0[1], v1[1];
+  sample<1>(ip, PlaneFetch{f.I0, f.NY, f.NX}, f.NZ, f.NY, f.NX, f.hZ, f.hY, f.hX, bc, cz, cy, cx, v0);
+  sample<1>(ip, PlaneFetch{f.I1, f.NY, f.NX}, f.NZ, f.NY, f.NX, f.hZ, f.hY, f.hX, bc, cz, cy, cx, v1);
+  d.I0[i] = v0[0];
+  d.I1[i] = v1[0];
+}
+
+// Per-level constants (tvl13d.py:179-192): G = grad I1 (Nabla3D_CD.forward)
+// packed with I1 as the gather source J, and the anisotropic weight and unit
+// normal (anisotropicDifferentialOps.cu:90-141) as SN.
+__global__ __launch_bounds__(kBlock) void k_level_setup(const LevelDesc* __restrict__ descs, const int* __restrict__ block_start, int ninst,
+                              int aniso, float alpha, float beta, int bc) {
+  SVS_BATCH_PROLOGUE(descs);
+  const float* I1 = d.I1;
+  const int64_t sy = d.NX, sz = (int64_t)d.NY * d.NX;
+  const float b0 = I1[i];
+  const float gx = cd_fwd(x > 0 ? I1[i - 1] : 0.0f, b0, x < d.NX - 1 ? I1[i + 1] : 0.0f, x, d.NX, d.hX, bc);
+  const float gy = cd_fwd(y > 0 ? I1[i - sy] : 0.0f, b0, y < d.NY - 1 ? I1[i + sy] : 0.0f, y, d.NY, d.hY, bc);
+  const float gz = cd_fwd(z > 0 ? I1[i - sz] : 0.0f, b0, z < d.NZ - 1 ? I1[i + sz] : 0.0f, z, d.NZ, d.hZ, bc);
+  d.J[i] = make_float4(b0, gx, gy, gz);
+  if (aniso) {
+    const float g[3] = {gx, gy, gz};
+    float ns = 0.0f;
+    for (int k = 0; k < 3; ++k) ns += g[k] * g[k];
+    const float nrm = sqrtf(ns);
+    float4 sn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (nrm > 0.0f) sn = make_float4(svs_expf(-alpha * svs_powf(nrm, beta)), gx / nrm, gy / nrm, gz / nrm);
+    d.SN[i] = sn;
+  }
+}
+
+// Zero start (tvl13d.py:116-118) or caller-provided u, p (used when the
+// pyramid has a single level, tvl13d.py:104-105) in SoA layout.
+__global__ __launch_bounds__(kBlock) void k_init_state(const LevelDesc* __restrict__ descs, const int* __restrict__ block_start, int ninst,
+                             int from_input) {
+  SVS_BATCH_PROLOGUE(descs);
+  (void)z; (void)y; (void)x;
+  const int64_t N = d.N;
+  for (int c = 0; c < 3; ++c) d.u[0][c * N + i] = from_input ? d.u_in[3 * i + c] : 0.0f;
+  for (int k = 0; k < 9; ++k) d.p[0][k * N + i] = from_input ? d.p_in[9 * i + k] : 0.0f;
+}
+
+struct Float4Fetch {
+  const float4* v;
+  int NY, NX;
+  __device__ void operator()(int z, int y, int x, float* o) const {
+    const float4 t = v[((int64_t)z * NY + y) * NX + x];
+    o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+  }
+};
+
+// Linearisation of the data term at the current flow (tvl13d.py:198-201):
+// I1w = warp(I1, u), g = warp(grad I1, u), rho_c = I1w - sum(u * g) - I0.
+// The four fields share the 64-tap geometry; each is evaluated exactly as the
+// scalar / vector-field warps do it.
+__global__ __launch_bounds__(kBlock) void k_linearise(const LevelDesc* __restrict__ descs, const int* __restrict__ block_start, int ninst,
+                            int parity, int ip, int bc) {
+  SVS_BATCH_PROLOGUE(descs);
+  const int64_t N = d.N;
+  const float* u = d.u[parity];
+  const float u0 = u[i], u1 = u[N + i], u2 = u[2 * N + i];
+  const float cx = (float)x * d.hX + u0;
+  const float cy = (float)y * d.hY + u1;
+  const float cz = (float)z * d.hZ + u2;
+  float v[4];
+  sample<4>(ip, Float4Fetch{d.J, d.NY, d.NX}, d.NZ, d.NY, d.NX, d.hZ, d.hY, d.hX, bc, cz, cy, cx, v);
+  const float s = (u0 * v[1] + u1 * v[2]) + u2 * v[3];
+  const float rc = (v[0] - s) - d.I0[i];
+  d.RG[i] = make_float4(rc, v[1], v[2], v[3]);
+}
+
+struct SoAFetch3 {
+  const float* v;
+  int64_t N;
+  int NY, NX;
+  __device__ void operator()(int z, int y, int x, float* o) const {
+    const int64_t k = ((int64_t)z * NY + y) * NX + x;
+    o[0] = v[k]; o[1] = v[N + k]; o[2] = v[2 * N + k];
+  }
+};
+
+// Coarse -> fine transfer (tvl13d.py:145-158): u by vector-field resampling
+// then component c scaled by L_c,fine / L_c,coarse; p by matrix-field
+// resampling, unscaled.
+__global__ __launch_bounds__(kBlock) void k_prolong_state(const LevelDesc* __restrict__ coarse,
+                                                          const LevelDesc* __restrict__ fine,
+                                                          const int* __restrict__ block_start, int ninst,
+                                                          int parity_c, int ip, int bc) {
+  SVS_BATCH_PROLOGUE(fine);
+  const LevelDesc& c = coarse[inst_];
+  const float cxp = (float)x * d.hX;
+  const float cx = cxp * c.LX / d.LX;
+  const float cyp = (float)y * d.hY;
+  const float cy = cyp * c.LY / d.LY;
+  const float czp = (float)z * d.hZ;
+  const float cz = czp * c.LZ / d.LZ;
+  const int64_t Nf = d.N;
+  float v[3];
+  sample<3>(ip, SoAFetch3{c.u[parity_c], c.N, c.NY, c.NX}, c.NZ, c.NY, c.NX, c.hZ, c.hY, c.hX, bc, cz, cy, cx, v);
+  d.u[0][i] = v[0] * d.upX;
+  d.u[0][Nf + i] = v[1] * d.upY;
+  d.u[0][2 * Nf + i] = v[2] * d.upZ;
+  for (int r = 0; r < 3; ++r) {
+    sample<3>(ip, SoAFetch3{c.p[parity_c] + 3 * r * c.N, c.N, c.NY, c.NX}, c.NZ, c.NY, c.NX, c.hZ, c.hY, c.hX, bc,
+              cz, cy, cx, v);
+    for (int k = 0; k < 3; ++k) d.p[0][(3 * r + k) * Nf + i] = v[k];
+  }
+}
+
+// SoA state -> the caller's [Z,Y,X,3] flow and [Z,Y,X,3,3] dual.
+__global__ __launch_bounds__(kBlock) void k_store_output(const LevelDesc* __restrict__ descs, const int* __restrict__ block_start, int ninst,
+                               int parity) {
+  SVS_BATCH_PROLOGUE(descs);
+  (void)z; (void)y; (void)x;
+  const int64_t N = d.N;
+  if (d.u_out)
+    for (int c = 0; c < 3; ++c) d.u_out[3 * i + c] = d.u[parity][c * N + i];
+  if (d.p_out)
+    for (int k = 0; k < 9; ++k) d.p_out[9 * i + k] = d.p[parity][k * N + i];
+}
+
+// ------------------------------------------------------ fused PD sweep
+// One primal-dual iteration (tvl13d.py:213-253) in a single pass:
+//   p' = proj_{|.|<=w_tv}(p + sigma * D grad u)                (dual ascent + prox)
+//   u' = prox_L1(u - tau * grad^T (D p'))  with rho at u      (primal descent + prox)
+// (the reference sets z = u after every step, so the dual uses grad u).
+//
+// A work-group owns an (x, y) tile of BX x BY columns including a one-voxel
+// ring and marches along z over [z0, z1).  At plane k it computes p' and
+// r = D p' on the whole extended tile (the ring is recomputed by the
+// neighbouring tile as well), then the primal update of plane k-1 from r of
+// planes k-2..k.  LDS holds u of plane k with a second ring (for the x/y
+// derivatives of the ring) and the x/y derivative columns of r for planes
+// k-1 and k; z-neighbours live in registers of the owning thread.  One
+// barrier per plane.
+struct alignas(16) SweepTile {
+  int inst, x0, y0, zr;  // zr = z0 | (z1 << 16)
+};
+
+template <int BC, bool ANISO>
+__global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ descs,
+                                                const SweepTile* __restrict__ tiles, int parity, float sigma,
+                                                float tau, float factor, float wtv, int wtv_zero) {
+  extern __shared__ float lds[];
+  const SweepTile t = tiles[blockIdx.x];
+  const LevelDesc& d = descs[t.inst];
+  const int NZ = d.NZ, NY = d.NY, NX = d.NX;
+  const int64_t N = d.N;
+  const float hZ = d.hZ, hY = d.hY, hX = d.hX;
+  const int BX = blockDim.x, BY = blockDim.y;
+  const int ex = threadIdx.x, ey = threadIdx.y;
+  const int x = t.x0 - 1 + ex, y = t.y0 - 1 + ey;
+  const int z0 = t.zr & 0xffff, z1 = t.zr >> 16;
+  const bool valid = x >= 0 && x < NX && y >= 0 && y < NY;
+  const bool interior = valid && ex >= 1 && ex <= BX - 2 && ey >= 1 && ey <= BY - 2;
+
+  const int UW = BX + 2, UH = BY + 2, UPL = UW * UH, RPL = BX * BY;
+  float* U = lds;                 // [2][3][UPL]
+  float* R = lds + 6 * UPL;       // [2][6][RPL]
+  const int uo = (ey + 1) * UW + (ex + 1);
+  const int ro = ey * BX + ex;
+  const int tid = ey * BX + ex;
+  const int nring = 2 * UW + 2 * BY;
+
+  const float* __restrict__ uin = d.u[parity];
+  float* __restrict__ uout = d.u[parity ^ 1];
+  const float* __restrict__ pin = d.p[parity];
+  float* __restrict__ pout = d.p[parity ^ 1];
+  const int64_t col = (int64_t)y * NX + x;
+  const int64_t plane = (int64_t)NY * NX;
+
+  const int kstart = z0 > 0 ? z0 - 1 : 0;
+  const int kend = z1 < NZ ? z1 : NZ;  // the step k == NZ only finishes plane NZ-1
+
+  float um[3] = {0.0f, 0.0f, 0.0f}, uc[3] = {0.0f, 0.0f, 0.0f}, up[3];
+  if (valid) {
+    if (kstart >= 1)
+      for (int c = 0; c < 3; ++c) um[c] = uin[c * N + (kstart - 1) * plane + col];
+    for (int c = 0; c < 3; ++c) uc[c] = uin[c * N + kstart * plane + col];
+  }
+  float r2a[3] = {0.0f, 0.0f, 0.0f}, r2b[3] = {0.0f, 0.0f, 0.0f}, r2c[3];
+
+  for (int k = kstart; k <= kend; ++k) {
+    const int buf = k & 1;
+    for (int c = 0; c < 3; ++c) up[c] = (valid && k + 1 < NZ) ? uin[c * N + (int64_t)(k + 1) * plane + col] : 0.0f;
+    if (k < NZ) {
+      float* Ub = U + buf * 3 * UPL;
+      for (int c = 0; c < 3; ++c) Ub[c * UPL + uo] = uc[c];
+      for (int r = tid; r < nring; r += RPL) {
+        int ux, uy;
+        if (r < 2 * UW) {
+          uy = r < UW ? 0 : UH - 1;
+          ux = r < UW ? r : r - UW;
+        } else {
+          const int rr = r - 2 * UW;
+          uy = 1 + (rr >> 1);
+          ux = (rr & 1) ? UW - 1 : 0;
+        }
+        const int gx = t.x0 - 2 + ux, gy = t.y0 - 2 + uy;
+        const bool ok = gx >= 0 && gx < NX && gy >= 0 && gy < NY;
+        const int64_t gi = (int64_t)k * plane + (int64_t)gy * NX + gx;
+        for (int c = 0; c < 3; ++c) Ub[c * UPL + uy * UW + ux] = ok ? uin[c * N + gi] : 0.0f;
+      }
+    }
+    __syncthreads();
+
+    // ---- dual step at plane k on the extended tile
+    float* Rb = R + buf * 6 * RPL;
+    if (valid && k < NZ) {
+      const float* Ub = U + buf * 3 * UPL;
+      const int64_t vi = (int64_t)k * plane + col;
+      float s = 0.0f, n[3] = {0.0f, 0.0f, 0.0f}, t1[3] = {0.0f, 0.0f, 0.0f};
+      if (ANISO) {
+        const float4 sn = d.SN[vi];
+        s = sn.x; n[0] = sn.y; n[1] = sn.z; n[2] = sn.w;
+        tangent1(n, t1);
+      }
+      const bool store_p = interior && k >= z0 && k < z1;
+      for (int c = 0; c < 3; ++c) {
+        const float* Uc = Ub + c * UPL;
+        const float b0 = Uc[uo];
+        float q[3];
+        q[0] = cd_fwd(Uc[uo - 1], b0, Uc[uo + 1], x, NX, hX, BC);
+        q[1] = cd_fwd(Uc[uo - UW], b0, Uc[uo + UW], y, NY, hY, BC);
+        q[2] = cd_fwd(um[c], b0, up[c], k, NZ, hZ, BC);
+        float dq[3];
+        if (ANISO) aniso_row(s, n, t1, q, dq);
+        else { dq[0] = q[0]; dq[1] = q[1]; dq[2] = q[2]; }
+        float pv[3];
+        for (int e = 0; e < 3; ++e) {
+          const float inc = sigma * dq[e];
+          pv[e] = pin[(3 * c + e) * N + vi] + inc;
+        }
+        prox_dual_row(pv, wtv);
+        if (wtv_zero) { pv[0] = 0.0f; pv[1] = 0.0f; pv[2] = 0.0f; }
+        if (store_p)
+          for (int e = 0; e < 3; ++e) pout[(3 * c + e) * N + vi] = pv[e];
+        float rr[3];
+        if (ANISO) aniso_row(s, n, t1, pv, rr);
+        else { rr[0] = pv[0]; rr[1] = pv[1]; rr[2] = pv[2]; }
+        Rb[(2 * c) * RPL + ro] = rr[0];
+        Rb[(2 * c + 1) * RPL + ro] = rr[1];
+        r2c[c] = rr[2];
+      }
+    } else {
+      for (int c = 0; c < 3; ++c) {
+        Rb[(2 * c) * RPL + ro] = 0.0f;
+        Rb[(2 * c + 1) * RPL + ro] = 0.0f;
+        r2c[c] = 0.0f;
+      }
+    }
+
+    // ---- primal step at plane k-1 on the tile interior
+    const int km = k - 1;
+    if (interior && km >= z0 && km < z1) {
+      const float* Rp = R + (km & 1) * 6 * RPL;
+      const int64_t vi = (int64_t)km * plane + col;
+      const float4 rg = d.RG[vi];
+      const float g[3] = {rg.y, rg.z, rg.w};
+      float v[3];
+      for (int c = 0; c < 3; ++c) {
+        const float* Rx = Rp + (2 * c) * RPL;
+        const float* Ry = Rp + (2 * c + 1) * RPL;
+        const float ax = cd_adj(Rx[ro - 1], Rx[ro], Rx[ro + 1], x, NX, BC);
+        const float ay = cd_adj(Ry[ro - BX], Ry[ro], Ry[ro + BX], y, NY, BC);
+        const float az = cd_adj(r2a[c], r2b[c], r2c[c], km, NZ, BC);
+        const float div = ax / hX + ay / hY + az / hZ;
+        const float step = tau * div;
+        v[c] = um[c] - step;
+      }
+      const float sm = (um[0] * g[0] + um[1] * g[1]) + um[2] * g[2];
+      const float rho = rg.x + sm;
+      prox_primal_point(v, rho, g, factor);
+      for (int c = 0; c < 3; ++c) uout[c * N + vi] = v[c];
+    }
+
+    for (int c = 0; c < 3; ++c) {
+      r2a[c] = r2b[c];
+      r2b[c] = r2c[c];
+      um[c] = uc[c];
+      uc[c] = up[c];
+    }
+  }
+}
+
+// ============================================================ host plan
+struct LevelPlan {
+  std::vector<LevelDesc> host;   // per instance
+  LevelDesc* dev = nullptr;      // table in the workspace
+  int* block_start = nullptr;    // per instance, + total
+  int nblocks = 0;
+  SweepTile* tiles = nullptr;
+  int ntiles = 0;
+  int bx = 0, by = 0, cz = 0;
+};
+
+}  // namespace svs
+
+struct svs_plan {
+  svs_tvl1_params P;
+  int ninst = 0;
+  int S = 0;
+  std::vector<int> dims;                  // 3 per instance (NZ, NY, NX)
+  std::vector<svs::LevelPlan> levels;
+  std::vector<float> sig, tau, factor;    // per iteration, rounded to float
+  std::vector<float> blur_w;              // K^3 weights (host copy)
+  float* blur_dev = nullptr;
+  char* ws = nullptr;
+  size_t ws_bytes = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t capture_stream = nullptr;
+  bool tables_dirty = true;
+};
+
+namespace svs {
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// numDofs meshes use L = N - 1 (flow_utils.py:48-71); h = L / (N - 1) in float.
+static void level_geometry(const svs_tvl1_params& P, int NZ, int NY, int NX, LevelDesc& d) {
+  d.NZ = NZ; d.NY = NY; d.NX = NX;
+  if (P.length_fixed) {
+    d.LZ = P.LZ; d.LY = P.LY; d.LX = P.LX;
+  } else {
+    d.LZ = (float)(NZ - 1); d.LY = (float)(NY - 1); d.LX = (float)(NX - 1);
+  }
+  d.hZ = d.LZ / (float)(NZ - 1); d.hY = d.LY / (float)(NY - 1); d.hX = d.LX / (float)(NX - 1);
+  d.N = (int64_t)NZ * NY * NX;
+}
+
+// Sweep launch shape for a level: (BX, BY) threads per work-group and the
+// z-chunk, chosen so the launch has enough work-groups to fill 256 CUs.
+static void choose_sweep_shape(const svs_tvl1_params& P, LevelPlan& L, int ninst) {
+  int maxNX = 0, maxNY = 0, maxNZ = 0;
+  int64_t vox = 0;
+  for (auto& d : L.host) {
+    maxNX = std::max(maxNX, d.NX); maxNY = std::max(maxNY, d.NY); maxNZ = std::max(maxNZ, d.NZ);
+    vox += d.N;
+  }
+  int bx = P.sweep_bx, by = P.sweep_by, cz = P.sweep_cz;
+  if (bx <= 0) bx = std::min(maxNX + 2, 66);
+  if (by <= 0) {
+    by = std::max(3, std::min(512 / bx, maxNY + 2));
+    if (bx * by > 1024) by = 1024 / bx;
+  }
+  if (cz <= 0) {
+    // Aim for >= 1024 work-groups when the level is large enough.
+    const int tx = bx - 2, ty = by - 2;
+    int64_t cols = 0;
+    for (auto& d : L.host) cols += (int64_t)((d.NX + tx - 1) / tx) * ((d.NY + ty - 1) / ty);
+    cz = maxNZ;
+    while (cz > 4 && cols * ((maxNZ + cz - 1) / cz) < 1024) cz = (cz + 1) / 2;
+  }
+  (void)ninst; (void)vox;
+  L.bx = bx; L.by = by; L.cz = cz;
+}
+
+static size_t sweep_lds_bytes(int bx, int by) {
+  return sizeof(float) * (6 * (size_t)(bx + 2) * (by + 2) + 12 * (size_t)bx * by);
+}
+
+}  // namespace svs
+
+using namespace svs;
+
+// Workspace layout, computed identically by the sizing and the plan builder.
+static int plan_layout(svs_plan* pl, char* base, size_t* total_out) {
+  const svs_tvl1_params& P = pl->P;
+  const int S = pl->S;
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> char* {
+    off = align_up(off, 256);
+    char* p = base ? base + off : nullptr;
+    off += bytes;
+    return p;
+  };
+  pl->levels.assign(S, LevelPlan());
+  for (int s = 0; s < S; ++s) pl->levels[s].host.resize(pl->ninst);
+  for (int k = 0; k < pl->ninst; ++k) {
+    int nz = pl->dims[3 * k], ny = pl->dims[3 * k + 1], nx = pl->dims[3 * k + 2];
+    const int64_t N0 = (int64_t)nz * ny * nx;
+    float4* J = (float4*)take(16 * N0);
+    float4* SN = (float4*)take(16 * N0);
+    float4* RG = (float4*)take(16 * N0);
+    for (int s = 0; s < S; ++s) {
+      if (s > 0) { nz = (nz + 1) / 2; ny = (ny + 1) / 2; nx = (nx + 1) / 2; }
+      LevelDesc& d = pl->levels[s].host[k];
+      std::memset(&d, 0, sizeof(d));
+      level_geometry(P, nz, ny, nx, d);
+      const int64_t N = d.N;
+      d.J = J; d.SN = SN; d.RG = RG;
+      if (s > 0 || P.use_blur) {
+        d.I0 = (float*)take(4 * N);
+        d.I1 = (float*)take(4 * N);
+      }
+      for (int b = 0; b < 2; ++b) {
+        d.u[b] = (float*)take(12 * N);
+        d.p[b] = (float*)take(36 * N);
+      }
+    }
+  }
+  // Device tables.
+  for (int s = 0; s < S; ++s) {
+    LevelPlan& L = pl->levels[s];
+    choose_sweep_shape(P, L, pl->ninst);
+    L.dev = (LevelDesc*)take(sizeof(LevelDesc) * pl->ninst);
+    L.block_start = (int*)take(sizeof(int) * (pl->ninst + 1));
+    const int tx = L.bx - 2, ty = L.by - 2;
+    int ntiles = 0;
+    for (auto& d : L.host)
+      ntiles += ((d.NX + tx - 1) / tx) * ((d.NY + ty - 1) / ty) * ((d.NZ + L.cz - 1) / L.cz);
+    L.ntiles = ntiles;
+    L.tiles = (SweepTile*)take(sizeof(SweepTile) * ntiles);
+    int nb = 0;
+    for (auto& d : L.host) nb += grid_for(d.N, kBlock);
+    L.nblocks = nb;
+  }
+  if (P.use_blur) pl->blur_dev = (float*)take(sizeof(float) * pl->blur_w.size());
+  *total_out = align_up(off, 256);
+  return 0;
+}
+
+static int validate_params(const svs_tvl1_params* P) {
+  SVS_REQUIRE(P, "null params");
+  SVS_REQUIRE(P->num_scales >= 1 && P->num_scales <= 16, "num_scales must be in [1,16]");
+  SVS_REQUIRE(P->max_warps >= 0 && P->max_outer_iterations >= 0, "negative warp/iteration count");
+  SVS_REQUIRE(valid_bc(P->boundary) && valid_ip(P->interpolation), "invalid interpolation/boundary type");
+  SVS_REQUIRE(!P->use_blur || (P->blur_K >= 1 && (P->blur_K & 1) && P->blur_weights),
+              "blur enabled but kernel size/weights invalid");
+  return 0;
+}
+
+static int validate_dims(const svs_tvl1_params* P, int ninst, const int* dims) {
+  SVS_REQUIRE(ninst >= 1 && dims, "need at least one instance");
+  for (int k = 0; k < ninst; ++k) {
+    int nz = dims[3 * k], ny = dims[3 * k + 1], nx = dims[3 * k + 2];
+    SVS_REQUIRE_DIMS(nz, ny, nx);
+    SVS_REQUIRE(nz < 32768, "NZ must be < 32768");
+    for (int s = 0; s < P->num_scales; ++s) {
+      if (s > 0) { nz = (nz + 1) / 2; ny = (ny + 1) / 2; nx = (nx + 1) / 2; }
+      SVS_REQUIRE(nz >= 4 && ny >= 4 && nx >= 4,
+                  "instance %d: pyramid level %d is %dx%dx%d; every level needs >= 4 voxels per axis", k, s, nz,
+                  ny, nx);
+    }
+  }
+  return 0;
+}
+
+static void fill_schedule(svs_plan* pl) {
+  const svs_tvl1_params& P = pl->P;
+  const int K = P.max_outer_iterations;
+  pl->sig.resize(K); pl->tau.resize(K); pl->factor.resize(K);
+  // tvl13d.py:208-246: Python floats, reset per warp, rounded to float at use.
+  double sg = P.sigma, ta = P.tau;
+  for (int n = 0; n < K; ++n) {
+    pl->sig[n] = (float)sg;
+    pl->tau[n] = (float)ta;
+    pl->factor[n] = (float)ta * P.weight_matching;  // opticalFlowOps.cu:442, FP32 product
+    if (P.pd_type == 2) {
+      const double th = 1. / std::sqrt(1. + 2. * P.gamma * ta);
+      ta *= th;
+      sg /= th;
+    }
+  }
+  for (int s = 0; s + 1 < pl->S; ++s)
+    for (int k = 0; k < pl->ninst; ++k) {
+      // tvl13d.py:151-153: ratio of one instance's FP32 lengths in Python
+      // floats, applied as a float; (N_fine - 1)/(N_coarse - 1) for numDofs.
+      LevelDesc& f = pl->levels[s].host[k];
+      const LevelDesc& c = pl->levels[s + 1].host[k];
+      f.upX = (float)((double)f.LX / (double)c.LX);
+      f.upY = (float)((double)f.LY / (double)c.LY);
+      f.upZ = (float)((double)f.LZ / (double)c.LZ);
+    }
+}
+
+static int upload_tables(svs_plan* pl, hipStream_t st) {
+  for (int s = 0; s < pl->S; ++s) {
+    LevelPlan& L = pl->levels[s];
+    SVS_HIP(hipMemcpyAsync(L.dev, L.host.data(), sizeof(LevelDesc) * pl->ninst, hipMemcpyHostToDevice, st));
+  }
+  SVS_HIP(hipStreamSynchronize(st));
+  pl->tables_dirty = false;
+  return 0;
+}
+
+static int upload_static(svs_plan* pl, hipStream_t st) {
+  for (int s = 0; s < pl->S; ++s) {
+    LevelPlan& L = pl->levels[s];
+    std::vector<int> bs(pl->ninst + 1);
+    int acc = 0;
+    for (int k = 0; k < pl->ninst; ++k) { bs[k] = acc; acc += grid_for(L.host[k].N, kBlock); }
+    bs[pl->ninst] = acc;
+    std::vector<SweepTile> tiles;
+    tiles.reserve(L.ntiles);
+    const int tx = L.bx - 2, ty = L.by - 2;
+    for (int k = 0; k < pl->ninst; ++k) {
+      const LevelDesc& d = L.host[k];
+      for (int z0 = 0; z0 < d.NZ; z0 += L.cz)
+        for (int y0 = 0; y0 < d.NY; y0 += ty)
+          for (int x0 = 0; x0 < d.NX; x0 += tx) {
+            const int z1 = std::min(d.NZ, z0 + L.cz);
+            tiles.push_back(SweepTile{k, x0, y0, z0 | (z1 << 16)});
+          }
+    }
+    SVS_HIP(hipMemcpyAsync(L.block_start, bs.data(), sizeof(int) * bs.size(), hipMemcpyHostToDevice, st));
+    SVS_HIP(hipMemcpyAsync(L.tiles, tiles.data(), sizeof(SweepTile) * tiles.size(), hipMemcpyHostToDevice, st));
+    SVS_HIP(hipStreamSynchronize(st));
+  }
+  if (pl->P.use_blur) {
+    SVS_HIP(hipMemcpyAsync(pl->blur_dev, pl->blur_w.data(), sizeof(float) * pl->blur_w.size(),
+                           hipMemcpyHostToDevice, st));
+    SVS_HIP(hipStreamSynchronize(st));
+  }
+  return 0;
+}
+
+template <int BC, bool ANISO>
+static void launch_sweep_t(const LevelPlan& L, int parity, float sg, float ta, float fac, float wtv, int wz,
+                           hipStream_t st) {
+  const size_t lds = sweep_lds_bytes(L.bx, L.by);
+  hipLaunchKernelGGL((k_sweep<BC, ANISO>), dim3(L.ntiles), dim3(L.bx, L.by), lds, st, L.dev, L.tiles, parity, sg,
+                     ta, fac, wtv, wz);
+}
+
+static void launch_sweep(const svs_plan* pl, const LevelPlan& L, int parity, int n, hipStream_t st) {
+  const svs_tvl1_params& P = pl->P;
+  const float sg = pl->sig[n], ta = pl->tau[n], fac = pl->factor[n];
+  const float wtv = P.weight_tv;
+  const int wz = P.weight_tv == 0.0f;
+  const bool an = P.use_aniso != 0;
+  switch (P.boundary * 2 + (an ? 1 : 0)) {
+    case 0: launch_sweep_t<BD_NEAREST, false>(L, parity, sg, ta, fac, wtv, wz, st); break;
+    case 1: launch_sweep_t<BD_NEAREST, true>(L, parity, sg, ta, fac, wtv, wz, st); break;
+    case 2: launch_sweep_t<BD_MIRROR, false>(L, parity, sg, ta, fac, wtv, wz, st); break;
+    case 3: launch_sweep_t<BD_MIRROR, true>(L, parity, sg, ta, fac, wtv, wz, st); break;
+    case 4: launch_sweep_t<BD_REFLECT, false>(L, parity, sg, ta, fac, wtv, wz, st); break;
+    default: launch_sweep_t<BD_REFLECT, true>(L, parity, sg, ta, fac, wtv, wz, st); break;
+  }
+}
+
+// Dynamic LDS above 64 KiB must be enabled per kernel.
+static int allow_large_lds(const svs_plan* pl) {
+  size_t need = 0;
+  for (const LevelPlan& L : pl->levels) need = std::max(need, sweep_lds_bytes(L.bx, L.by));
+  if (need <= 64 * 1024) return 0;
+  const int v = (int)need;
+  SVS_HIP(hipFuncSetAttribute((const void*)k_sweep<BD_NEAREST, false>, hipFuncAttributeMaxDynamicSharedMemorySize, v));
+  SVS_HIP(hipFuncSetAttribute((const void*)k_sweep<BD_NEAREST, true>, hipFuncAttributeMaxDynamicSharedMemorySize, v));
+  SVS_HIP(hipFuncSetAttribute((const void*)k_sweep<BD_MIRROR, false>, hipFuncAttributeMaxDynamicSharedMemorySize, v));
+  SVS_HIP(hipFuncSetAttribute((const void*)k_sweep<BD_MIRROR, true>, hipFuncAttributeMaxDynamicSharedMemorySize, v));
+  SVS_HIP(hipFuncSetAttribute((const void*)k_sweep<BD_REFLECT, false>, hipFuncAttributeMaxDynamicSharedMemorySize, v));
+  SVS_HIP(hipFuncSetAttribute((const void*)k_sweep<BD_REFLECT, true>, hipFuncAttributeMaxDynamicSharedMemorySize, v));
+  return 0;
+}
+
+// Enqueues the complete solve of every instance on one stream.
+static int enqueue_solve(svs_plan* pl, hipStream_t st) {
+  const svs_tvl1_params& P = pl->P;
+  const int S = pl->S, n = pl->ninst;
+  const int ip = P.interpolation, bc = P.boundary;
+  if (P.use_blur) {
+    LevelPlan& L0 = pl->levels[0];
+    k_blur_pair<<<L0.nblocks, kBlock, 0, st>>>(L0.dev, L0.block_start, n, pl->blur_dev, P.blur_K);
+    SVS_LAUNCHED();
+  }
+  for (int s = 1; s < S; ++s) {
+    LevelPlan& L = pl->levels[s];
+    k_restrict<<<L.nblocks, kBlock, 0, st>>>(pl->levels[s - 1].dev, L.dev, L.block_start, n, ip, bc);
+    SVS_LAUNCHED();
+  }
+  int parity = 0;
+  for (int s = S - 1; s >= 0; --s) {
+    LevelPlan& L = pl->levels[s];
+    k_level_setup<<<L.nblocks, kBlock, 0, st>>>(L.dev, L.block_start, n, P.use_aniso, P.alpha, P.beta, bc);
+    SVS_LAUNCHED();
+    if (s == S - 1) {
+      const int from_input = (S == 1 && P.use_initial_state) ? 1 : 0;
+      k_init_state<<<L.nblocks, kBlock, 0, st>>>(L.dev, L.block_start, n, from_input);
+      SVS_LAUNCHED();
+    }
+    parity = 0;
+    for (int w = 0; w < P.max_warps; ++w) {
+      k_linearise<<<L.nblocks, kBlock, 0, st>>>(L.dev, L.block_start, n, parity, ip, bc);
+      SVS_LAUNCHED();
+      for (int it = 0; it < P.max_outer_iterations; ++it) {
+        launch_sweep(pl, L, parity, it, st);
+        SVS_LAUNCHED();
+        parity ^= 1;
+      }
+    }
+    if (s > 0) {
+      LevelPlan& F = pl->levels[s - 1];
+      k_prolong_state<<<F.nblocks, kBlock, 0, st>>>(L.dev, F.dev, F.block_start, n, parity, ip, bc);
+      SVS_LAUNCHED();
+    }
+  }
+  LevelPlan& L0 = pl->levels[0];
+  k_store_output<<<L0.nblocks, kBlock, 0, st>>>(L0.dev, L0.block_start, n, parity);
+  SVS_LAUNCHED();
+  return 0;
+}
+
+extern "C" {
+
+size_t svs_tvl1_workspace_bytes(const svs_tvl1_params* P, int ninst, const int* dims) {
+  if (validate_params(P) || validate_dims(P, ninst, dims)) return 0;
+  svs_plan tmp;
+  tmp.P = *P;
+  tmp.ninst = ninst;
+  tmp.S = P->num_scales;
+  tmp.dims.assign(dims, dims + 3 * ninst);
+  if (P->use_blur) tmp.blur_w.assign(P->blur_weights, P->blur_weights + (size_t)P->blur_K * P->blur_K * P->blur_K);
+  size_t total = 0;
+  plan_layout(&tmp, nullptr, &total);
+  return total;
+}
+
+int svs_tvl1_plan_create(const svs_tvl1_params* P, int ninst, const int* dims, void* workspace, size_t ws_bytes,
+                         void* stream, svs_plan** out) {
+  if (int rc = validate_params(P)) return rc;
+  if (int rc = validate_dims(P, ninst, dims)) return rc;
+  SVS_REQUIRE(out, "null plan output");
+  SVS_REQUIRE(workspace, "null workspace");
+  svs_plan* pl = new (std::nothrow) svs_plan();
+  SVS_REQUIRE(pl, "out of host memory");
+  pl->P = *P;
+  pl->ninst = ninst;
+  pl->S = P->num_scales;
+  pl->dims.assign(dims, dims + 3 * ninst);
+  if (P->use_blur) pl->blur_w.assign(P->blur_weights, P->blur_weights + (size_t)P->blur_K * P->blur_K * P->blur_K);
+  size_t need = 0;
+  plan_layout(pl, nullptr, &need);
+  if (ws_bytes < need) {
+    delete pl;
+    return fail(SVS_STATUS_INVALID, "workspace too small: %zu bytes given, %zu needed", ws_bytes, need);
+  }
+  pl->ws = (char*)workspace;
+  pl->ws_bytes = ws_bytes;
+  size_t total = 0;
+  plan_layout(pl, pl->ws, &total);
+  pl->P.blur_weights = nullptr;  // host copy kept in blur_w
+  for (int s = 0; s < pl->S; ++s) {
+    LevelPlan& L = pl->levels[s];
+    const size_t lds = sweep_lds_bytes(L.bx, L.by);
+    if (L.bx * L.by > 1024 || L.bx < 3 || L.by < 3 || lds > 160 * 1024) {
+      delete pl;
+      return fail(SVS_STATUS_INVALID, "sweep shape %dx%d not launchable", L.bx, L.by);
+    }
+  }
+  fill_schedule(pl);
+  if (int rc = allow_large_lds(pl)) { delete pl; return rc; }
+  hipStream_t st = (hipStream_t)stream;
+  if (int rc = upload_static(pl, st)) { delete pl; return rc; }
+  *out = pl;
+  return 0;
+}
+
+int svs_tvl1_plan_bind(svs_plan* pl, int k, const float* I0, const float* I1, const float* u_in,
+                       const float* p_in, float* u_out, float* p_out) {
+  SVS_REQUIRE(pl && k >= 0 && k < pl->ninst, "plan/instance index out of range");
+  SVS_REQUIRE(I0 && I1, "instance %d: null input frame", k);
+  LevelDesc& d = pl->levels[0].host[k];
+  d.src0 = I0;
+  d.src1 = I1;
+  if (!pl->P.use_blur) {
+    d.I0 = const_cast<float*>(I0);
+    d.I1 = const_cast<float*>(I1);
+  }
+  d.u_in = u_in;
+  d.p_in = p_in;
+  d.u_out = u_out;
+  d.p_out = p_out;
+  if (pl->S == 1 && pl->P.use_initial_state) SVS_REQUIRE(u_in && p_in, "instance %d: initial state required", k);
+  pl->tables_dirty = true;
+  return 0;
+}
+
+int svs_tvl1_plan_run(svs_plan* pl, int use_graph, void* stream) {
+  SVS_REQUIRE(pl, "null plan");
+  hipStream_t st = (hipStream_t)stream;
+  for (int k = 0; k < pl->ninst; ++k)
+    SVS_REQUIRE(pl->levels[0].host[k].src0, "instance %d has no bound input", k);
+  if (pl->tables_dirty)
+    if (int rc = upload_tables(pl, st)) return rc;
+  if (!use_graph) return enqueue_solve(pl, st);
+  if (!pl->exec) {
+    if (!pl->capture_stream) SVS_HIP(hipStreamCreateWithFlags(&pl->capture_stream, hipStreamNonBlocking));
+    SVS_HIP(hipStreamBeginCapture(pl->capture_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_solve(pl, pl->capture_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(pl->capture_stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(SVS_STATUS_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    pl->graph = g;
+    SVS_HIP(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
+  }
+  SVS_HIP(hipGraphLaunch(pl->exec, st));
+  return 0;
+}
+
+int svs_tvl1_plan_destroy(svs_plan* pl) {
+  if (!pl) return 0;
+  if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+  if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  if (pl->capture_stream) (void)hipStreamDestroy(pl->capture_stream);
+  delete pl;
+  return 0;
+}
+
+int svs_tvl1_plan_info(const svs_plan* pl, int level, int* out /* 8 ints */) {
+  SVS_REQUIRE(pl && level >= 0 && level < pl->S && out, "bad plan/level");
+  const LevelPlan& L = pl->levels[level];
+  out[0] = L.bx; out[1] = L.by; out[2] = L.cz; out[3] = L.ntiles; out[4] = L.nblocks;
+  out[5] = pl->ninst; out[6] = pl->S; out[7] = 0;
+  return 0;
+}
+
+// Enqueues `count` fused sweeps of one level on the state a previous run
+// left behind (ping-pong, iteration scalars cycling through the schedule).
+// Used to time the sweep kernel on its own.
+int svs_tvl1_plan_sweeps(svs_plan* pl, int level, int count, void* stream) {
+  SVS_REQUIRE(pl && level >= 0 && level < pl->S && count >= 0, "bad plan/level/count");
+  SVS_REQUIRE(!pl->tables_dirty, "run the plan once before timing its sweeps");
+  SVS_REQUIRE(pl->P.max_outer_iterations > 0, "no iterations configured");
+  hipStream_t st = (hipStream_t)stream;
+  int parity = 0;
+  for (int i = 0; i < count; ++i) {
+    launch_sweep(pl, pl->levels[level], parity, i % pl->P.max_outer_iterations, st);
+    SVS_LAUNCHED();
+    parity ^= 1;
+  }
+  return 0;
+}
+
+// One frame pair, enqueued directly (no graph): the plain drop-in.
+int svs_tvl1_solve(const svs_tvl1_params* P, const float* I0, const float* I1, int NZ, int NY, int NX,
+                   const float* u_in, const float* p_in, float* u_out, float* p_out, void* workspace,
+                   size_t ws_bytes, void* stream) {
+  const int dims[3] = {NZ, NY, NX};
+  svs_plan* pl = nullptr;
+  if (int rc = svs_tvl1_plan_create(P, 1, dims, workspace, ws_bytes, stream, &pl)) return rc;
+  int rc = svs_tvl1_plan_bind(pl, 0, I0, I1, u_in, p_in, u_out, p_out);
+  if (!rc) rc = svs_tvl1_plan_run(pl, 0, stream);
+  svs_tvl1_plan_destroy(pl);
+  return rc;
+}
+
+}  // extern "C"
