@@ -53,6 +53,7 @@ struct alignas(16) LevelDesc {
 };
 
 constexpr int kBlock = 256;
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, work-groups dealt round-robin
 
 // Maps a block of a flattened multi-instance launch to (instance, voxel).
 __device__ __forceinline__ int find_instance(const int* __restrict__ block_start, int ninst, int blk) {
@@ -279,47 +280,110 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   const int uo = (ey + 1) * UW + (ex + 1);
   const int ro = ey * BX + ex;
   const int tid = ey * BX + ex;
-  const int nring = 2 * UW + 2 * BY;
 
-  const float* __restrict__ uin = d.u[parity];
-  float* __restrict__ uout = d.u[parity ^ 1];
-  const float* __restrict__ pin = d.p[parity];
-  float* __restrict__ pout = d.p[parity ^ 1];
-  const int64_t col = (int64_t)y * NX + x;
+  // Global addressing: a wave-uniform plane base plus this thread's 32-bit
+  // in-plane offset (the host keeps NY*NX < 2^31).
   const int64_t plane = (int64_t)NY * NX;
+  const uint32_t col = valid ? (uint32_t)(y * NX + x) : 0u;
+  const SVS_GLOBAL float* __restrict__ uin = as_global((const float*)d.u[parity]);
+  SVS_GLOBAL float* __restrict__ uout = as_global(d.u[parity ^ 1]);
+  const SVS_GLOBAL float* __restrict__ pin = as_global((const float*)d.p[parity]);
+  SVS_GLOBAL float* __restrict__ pout = as_global(d.p[parity ^ 1]);
+  const SVS_GLOBAL float4* __restrict__ SNg = as_global((const float4*)d.SN);
+  const SVS_GLOBAL float4* __restrict__ RGg = as_global((const float4*)d.RG);
+
+  // This thread's position on the outer ring of the u tile (one per thread;
+  // the host guarantees 2*UW + 2*BY <= BX*BY).
+  const int nring = 2 * UW + 2 * BY;
+  const bool ring = tid < nring;
+  int ring_lds = 0;
+  uint32_t ring_col = 0;
+  bool ring_ok = false;
+  if (ring) {
+    int ux, uy;
+    if (tid < 2 * UW) {
+      uy = tid < UW ? 0 : UH - 1;
+      ux = tid < UW ? tid : tid - UW;
+    } else {
+      const int rr = tid - 2 * UW;
+      uy = 1 + (rr >> 1);
+      ux = (rr & 1) ? UW - 1 : 0;
+    }
+    const int gx = t.x0 - 2 + ux, gy = t.y0 - 2 + uy;
+    ring_ok = gx >= 0 && gx < NX && gy >= 0 && gy < NY;
+    ring_col = ring_ok ? (uint32_t)(gy * NX + gx) : 0u;
+    ring_lds = uy * UW + ux;
+  }
 
   const int kstart = z0 > 0 ? z0 - 1 : 0;
   const int kend = z1 < NZ ? z1 : NZ;  // the step k == NZ only finishes plane NZ-1
 
-  float um[3] = {0.0f, 0.0f, 0.0f}, uc[3] = {0.0f, 0.0f, 0.0f}, up[3];
+  // Registers: own-column u of planes k-1..k+1 (k+2 in flight), the dual and
+  // anisotropy of plane k (k+1 in flight), rho_c/g of plane k-1 (k in
+  // flight), the ring value of plane k (k+1 in flight) and the z-derivative
+  // column of r = D p' for planes k-2..k.
+  float um[3] = {0.0f, 0.0f, 0.0f}, uc[3] = {0.0f, 0.0f, 0.0f}, up[3] = {0.0f, 0.0f, 0.0f};
+  float pc[9], snc[4] = {0.0f, 0.0f, 0.0f, 0.0f}, rgc[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ringc[3] = {0.0f, 0.0f, 0.0f};
+  for (int e = 0; e < 9; ++e) pc[e] = 0.0f;
   if (valid) {
-    if (kstart >= 1)
-      for (int c = 0; c < 3; ++c) um[c] = uin[c * N + (kstart - 1) * plane + col];
-    for (int c = 0; c < 3; ++c) uc[c] = uin[c * N + kstart * plane + col];
+    if (kstart >= 1) {
+      const SVS_GLOBAL float* b = uin + (kstart - 1) * plane;
+      for (int c = 0; c < 3; ++c) um[c] = (b + c * N)[col];
+    }
+    const SVS_GLOBAL float* b0 = uin + kstart * plane;
+    for (int c = 0; c < 3; ++c) uc[c] = (b0 + c * N)[col];
+    if (kstart + 1 < NZ) {
+      const SVS_GLOBAL float* b1 = uin + (kstart + 1) * plane;
+      for (int c = 0; c < 3; ++c) up[c] = (b1 + c * N)[col];
+    }
+    const SVS_GLOBAL float* pb = pin + kstart * plane;
+    for (int e = 0; e < 9; ++e) pc[e] = (pb + e * N)[col];
+    if (ANISO) {
+      const svs_f4 sn = ld4(SNg + kstart * plane, col);
+      snc[0] = sn.x; snc[1] = sn.y; snc[2] = sn.z; snc[3] = sn.w;
+    }
+  }
+  if (ring_ok) {
+    const SVS_GLOBAL float* b = uin + kstart * plane;
+    for (int c = 0; c < 3; ++c) ringc[c] = (b + c * N)[ring_col];
   }
   float r2a[3] = {0.0f, 0.0f, 0.0f}, r2b[3] = {0.0f, 0.0f, 0.0f}, r2c[3];
 
   for (int k = kstart; k <= kend; ++k) {
     const int buf = k & 1;
-    for (int c = 0; c < 3; ++c) up[c] = (valid && k + 1 < NZ) ? uin[c * N + (int64_t)(k + 1) * plane + col] : 0.0f;
+    // ---- prefetch what step k+1 needs; consumed after this step's barrier
+    float un[3] = {0.0f, 0.0f, 0.0f}, pn[9], snn[4] = {0.0f, 0.0f, 0.0f, 0.0f}, rgn[4] = {0.0f, 0.0f, 0.0f, 0.0f},
+          ringn[3] = {0.0f, 0.0f, 0.0f};
+    for (int e = 0; e < 9; ++e) pn[e] = 0.0f;
+    if (valid) {
+      if (k + 2 < NZ) {
+        const SVS_GLOBAL float* b = uin + (k + 2) * plane;
+        for (int c = 0; c < 3; ++c) un[c] = (b + c * N)[col];
+      }
+      if (k + 1 < NZ) {
+        const SVS_GLOBAL float* pb = pin + (k + 1) * plane;
+        for (int e = 0; e < 9; ++e) pn[e] = (pb + e * N)[col];
+        if (ANISO) {
+          const svs_f4 sn = ld4(SNg + (k + 1) * plane, col);
+          snn[0] = sn.x; snn[1] = sn.y; snn[2] = sn.z; snn[3] = sn.w;
+        }
+      }
+    }
+    if (interior && k >= z0 && k < z1) {
+      const svs_f4 rg = ld4(RGg + k * plane, col);
+      rgn[0] = rg.x; rgn[1] = rg.y; rgn[2] = rg.z; rgn[3] = rg.w;
+    }
+    if (ring_ok && k + 1 < NZ) {
+      const SVS_GLOBAL float* b = uin + (k + 1) * plane;
+      for (int c = 0; c < 3; ++c) ringn[c] = (b + c * N)[ring_col];
+    }
+
+    // ---- u of plane k into LDS
     if (k < NZ) {
       float* Ub = U + buf * 3 * UPL;
       for (int c = 0; c < 3; ++c) Ub[c * UPL + uo] = uc[c];
-      for (int r = tid; r < nring; r += RPL) {
-        int ux, uy;
-        if (r < 2 * UW) {
-          uy = r < UW ? 0 : UH - 1;
-          ux = r < UW ? r : r - UW;
-        } else {
-          const int rr = r - 2 * UW;
-          uy = 1 + (rr >> 1);
-          ux = (rr & 1) ? UW - 1 : 0;
-        }
-        const int gx = t.x0 - 2 + ux, gy = t.y0 - 2 + uy;
-        const bool ok = gx >= 0 && gx < NX && gy >= 0 && gy < NY;
-        const int64_t gi = (int64_t)k * plane + (int64_t)gy * NX + gx;
-        for (int c = 0; c < 3; ++c) Ub[c * UPL + uy * UW + ux] = ok ? uin[c * N + gi] : 0.0f;
-      }
+      if (ring)
+        for (int c = 0; c < 3; ++c) Ub[c * UPL + ring_lds] = ringc[c];
     }
     __syncthreads();
 
@@ -327,14 +391,13 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
     float* Rb = R + buf * 6 * RPL;
     if (valid && k < NZ) {
       const float* Ub = U + buf * 3 * UPL;
-      const int64_t vi = (int64_t)k * plane + col;
       float s = 0.0f, n[3] = {0.0f, 0.0f, 0.0f}, t1[3] = {0.0f, 0.0f, 0.0f};
       if (ANISO) {
-        const float4 sn = d.SN[vi];
-        s = sn.x; n[0] = sn.y; n[1] = sn.z; n[2] = sn.w;
+        s = snc[0]; n[0] = snc[1]; n[1] = snc[2]; n[2] = snc[3];
         tangent1(n, t1);
       }
       const bool store_p = interior && k >= z0 && k < z1;
+      SVS_GLOBAL float* pob = pout + k * plane;
       for (int c = 0; c < 3; ++c) {
         const float* Uc = Ub + c * UPL;
         const float b0 = Uc[uo];
@@ -348,12 +411,12 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
         float pv[3];
         for (int e = 0; e < 3; ++e) {
           const float inc = sigma * dq[e];
-          pv[e] = pin[(3 * c + e) * N + vi] + inc;
+          pv[e] = pc[3 * c + e] + inc;
         }
         prox_dual_row(pv, wtv);
         if (wtv_zero) { pv[0] = 0.0f; pv[1] = 0.0f; pv[2] = 0.0f; }
         if (store_p)
-          for (int e = 0; e < 3; ++e) pout[(3 * c + e) * N + vi] = pv[e];
+          for (int e = 0; e < 3; ++e) (pob + (3 * c + e) * N)[col] = pv[e];
         float rr[3];
         if (ANISO) aniso_row(s, n, t1, pv, rr);
         else { rr[0] = pv[0]; rr[1] = pv[1]; rr[2] = pv[2]; }
@@ -373,9 +436,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
     const int km = k - 1;
     if (interior && km >= z0 && km < z1) {
       const float* Rp = R + (km & 1) * 6 * RPL;
-      const int64_t vi = (int64_t)km * plane + col;
-      const float4 rg = d.RG[vi];
-      const float g[3] = {rg.y, rg.z, rg.w};
+      const float g[3] = {rgc[1], rgc[2], rgc[3]};
       float v[3];
       for (int c = 0; c < 3; ++c) {
         const float* Rx = Rp + (2 * c) * RPL;
@@ -388,9 +449,10 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
         v[c] = um[c] - step;
       }
       const float sm = (um[0] * g[0] + um[1] * g[1]) + um[2] * g[2];
-      const float rho = rg.x + sm;
+      const float rho = rgc[0] + sm;
       prox_primal_point(v, rho, g, factor);
-      for (int c = 0; c < 3; ++c) uout[c * N + vi] = v[c];
+      SVS_GLOBAL float* uob = uout + km * plane;
+      for (int c = 0; c < 3; ++c) (uob + c * N)[col] = v[c];
     }
 
     for (int c = 0; c < 3; ++c) {
@@ -398,7 +460,11 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
       r2b[c] = r2c[c];
       um[c] = uc[c];
       uc[c] = up[c];
+      up[c] = un[c];
+      ringc[c] = ringn[c];
     }
+    for (int e = 0; e < 9; ++e) pc[e] = pn[e];
+    for (int e = 0; e < 4; ++e) { snc[e] = snn[e]; rgc[e] = rgn[e]; }
   }
 }
 
@@ -448,30 +514,45 @@ static void level_geometry(const svs_tvl1_params& P, int NZ, int NY, int NX, Lev
   d.N = (int64_t)NZ * NY * NX;
 }
 
-// Sweep launch shape for a level: (BX, BY) threads per work-group and the
-// z-chunk, chosen so the launch has enough work-groups to fill 256 CUs.
+// Sweep launch shape for a level: (BX, BY) threads per work-group (the tile
+// plus its one-voxel ring) and the z-chunk.  Large levels (a batch of many
+// volumes) use full-row tiles 8 rows high marching over long z-chunks; small
+// levels use narrow tiles and short chunks so that enough work-groups exist.
 static void choose_sweep_shape(const svs_tvl1_params& P, LevelPlan& L, int ninst) {
+  (void)ninst;
   int maxNX = 0, maxNY = 0, maxNZ = 0;
   int64_t vox = 0;
   for (auto& d : L.host) {
     maxNX = std::max(maxNX, d.NX); maxNY = std::max(maxNY, d.NY); maxNZ = std::max(maxNZ, d.NZ);
     vox += d.N;
   }
+  // Measured on MI355X (80^3 levels): a batch of large volumes runs best with
+  // ~40 x 10 interior tiles marching 40 planes; a lone small level with
+  // ~10 x 4 tiles and 2-plane chunks (it is latency-bound, so it wants many
+  // short work-groups).
+  const bool big = vox >= ((int64_t)8 << 20);
+  const bool tiny = vox < ((int64_t)1 << 20);
   int bx = P.sweep_bx, by = P.sweep_by, cz = P.sweep_cz;
-  if (bx <= 0) bx = std::min(maxNX + 2, 66);
+  if (bx <= 0) {
+    const int maxtx = big ? 64 : (tiny ? 10 : 20);
+    const int ntx = (maxNX + maxtx - 1) / maxtx;
+    bx = (maxNX + ntx - 1) / ntx + 2;
+  }
   if (by <= 0) {
-    by = std::max(3, std::min(512 / bx, maxNY + 2));
-    if (bx * by > 1024) by = 1024 / bx;
+    int ty = std::min(maxNY, big ? 10 : (tiny ? 4 : 10));
+    while (ty > 1 && bx * (ty + 2) > 1024) --ty;
+    by = ty + 2;
+    // one outer-ring position per thread
+    while (2 * (bx + 2) + 2 * by > bx * by) ++by;
   }
   if (cz <= 0) {
-    // Aim for >= 1024 work-groups when the level is large enough.
     const int tx = bx - 2, ty = by - 2;
     int64_t cols = 0;
     for (auto& d : L.host) cols += (int64_t)((d.NX + tx - 1) / tx) * ((d.NY + ty - 1) / ty);
+    const int target = big ? 1024 : 2048, minz = big ? 40 : (tiny ? 2 : 4);
     cz = maxNZ;
-    while (cz > 4 && cols * ((maxNZ + cz - 1) / cz) < 1024) cz = (cz + 1) / 2;
+    while (cz > minz && cols * ((maxNZ + cz - 1) / cz) < target) cz = (cz + 1) / 2;
   }
-  (void)ninst; (void)vox;
   L.bx = bx; L.by = by; L.cz = cz;
 }
 
@@ -611,8 +692,8 @@ static int upload_static(svs_plan* pl, hipStream_t st) {
     int acc = 0;
     for (int k = 0; k < pl->ninst; ++k) { bs[k] = acc; acc += grid_for(L.host[k].N, kBlock); }
     bs[pl->ninst] = acc;
-    std::vector<SweepTile> tiles;
-    tiles.reserve(L.ntiles);
+    std::vector<SweepTile> natural;
+    natural.reserve(L.ntiles);
     const int tx = L.bx - 2, ty = L.by - 2;
     for (int k = 0; k < pl->ninst; ++k) {
       const LevelDesc& d = L.host[k];
@@ -620,8 +701,22 @@ static int upload_static(svs_plan* pl, hipStream_t st) {
         for (int y0 = 0; y0 < d.NY; y0 += ty)
           for (int x0 = 0; x0 < d.NX; x0 += tx) {
             const int z1 = std::min(d.NZ, z0 + L.cz);
-            tiles.push_back(SweepTile{k, x0, y0, z0 | (z1 << 16)});
+            natural.push_back(SweepTile{k, x0, y0, z0 | (z1 << 16)});
           }
+    }
+    // Work-group b runs on XCD b % 8.  Deal contiguous runs of the natural
+    // order (neighbouring tiles of one volume) to one XCD each, so the halo
+    // rows a tile re-reads from its neighbours sit in the same L2.
+    std::vector<SweepTile> tiles;
+    tiles.reserve(natural.size());
+    {
+      const int nx = kXcds, n = (int)natural.size();
+      std::vector<int> start(nx + 1);
+      for (int g = 0; g <= nx; ++g) start[g] = (int)((int64_t)n * g / nx);
+      const int maxlen = (n + nx - 1) / nx;
+      for (int slot = 0; slot < maxlen; ++slot)
+        for (int g = 0; g < nx; ++g)
+          if (start[g] + slot < start[g + 1]) tiles.push_back(natural[start[g] + slot]);
     }
     SVS_HIP(hipMemcpyAsync(L.block_start, bs.data(), sizeof(int) * bs.size(), hipMemcpyHostToDevice, st));
     SVS_HIP(hipMemcpyAsync(L.tiles, tiles.data(), sizeof(SweepTile) * tiles.size(), hipMemcpyHostToDevice, st));
@@ -763,7 +858,8 @@ int svs_tvl1_plan_create(const svs_tvl1_params* P, int ninst, const int* dims, v
   for (int s = 0; s < pl->S; ++s) {
     LevelPlan& L = pl->levels[s];
     const size_t lds = sweep_lds_bytes(L.bx, L.by);
-    if (L.bx * L.by > 1024 || L.bx < 3 || L.by < 3 || lds > 160 * 1024) {
+    if (L.bx * L.by > 1024 || L.bx < 3 || L.by < 3 || lds > 160 * 1024 ||
+        2 * (L.bx + 2) + 2 * L.by > L.bx * L.by || L.cz < 1) {
       delete pl;
       return fail(SVS_STATUS_INVALID, "sweep shape %dx%d not launchable", L.bx, L.by);
     }

@@ -251,6 +251,21 @@ __device__ __forceinline__ void prox_primal_point(float v[3], float r, const flo
   for (int c = 0; c < 3; ++c) v[c] = v[c] + delta[c];
 }
 
+// Pointers read from a descriptor table are generic (flat) to the compiler;
+// casting them to the global address space lets it address with a wave-
+// uniform SGPR base plus a 32-bit per-lane offset.
+#define SVS_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ SVS_GLOBAL T* as_global(T* p) {
+  return (SVS_GLOBAL T*)p;
+}
+
+// 16-byte vector load from a global-address-space float4 array.
+typedef float svs_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ svs_f4 ld4(const SVS_GLOBAL float4* p, uint32_t i) {
+  return ((const SVS_GLOBAL svs_f4*)p)[i];
+}
+
 // Linear voxel index -> (z, y, x).
 __device__ __forceinline__ void unflatten(int64_t i, int NY, int NX, int& z, int& y, int& x) {
   x = (int)(i % NX);
