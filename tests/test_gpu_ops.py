@@ -176,3 +176,30 @@ def test_invalid_inputs_raise(dev):
         op.forwardVectorField(torch.zeros(8, 8, 8, device=dev))  # wrong rank
     with pytest.raises(RuntimeError):
         OF.Warping3D(m, 2, 1).forward(torch.zeros(8, 8, 8, device=dev), torch.zeros(8, 8, 7, 3, device=dev))
+
+
+def test_gpu_reproduces_golden_ops(dev):
+    import os
+    g = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", "ops_10x11x12.npz"), allow_pickle=False))
+    m = mesh(g["u"].shape)
+    assert np.array_equal(h(OF.Warping3D(m, 2, 1).forward(g_(g["u"], dev), g_(g["phi"], dev))), g["warp_cubic_mirror"])
+    assert np.array_equal(h(OF.Warping3D(m, 1, 2).forwardVectorField(g_(g["v"], dev), g_(g["phi"], dev))),
+                          g["warp_vf_linear_reflect"])
+    op = OF.Nabla3D_CD(m, 1)
+    assert np.array_equal(h(op.forwardVectorField(g_(g["v"], dev))), g["nabla_fwd_vf_mirror"])
+    assert np.array_equal(h(op.backwardVectorField(g_(g["q"], dev))), g["nabla_bwd_vf_mirror"])
+    an = OF.AnisotropicNabla3D(m, 5.0, 0.5)
+    outs = [h(t) for t in an.computeTangentVecs(g_(g["G"], dev))]
+    for a, k in zip(outs, ("aniso_s", "aniso_n", "aniso_t1", "aniso_t2")):
+        assert np.array_equal(a, g[k])
+    args = [g_(g[k], dev) for k in ("aniso_s", "aniso_n", "aniso_t1", "aniso_t2")]
+    assert np.array_equal(h(an.forwardVectorField(g_(g["q"], dev), *args)), g["aniso_apply"])
+    assert np.array_equal(h(OF.TVL1OF3D_proxPrimal(g_(g["v"], dev), 0.37, 1.0, g_(g["rho"], dev), g_(g["G"], dev), m)),
+                          g["prox_primal"])
+    assert np.array_equal(h(OF.TVL1OF3D_proxDual(g_(g["q"] * np.float32(0.03), dev), 0.5, 0.025, m)), g["prox_dual"])
+    pm = OF.Prolongation3D(OF.MeshInfo3D(10, 11, 12, 9, 10, 11), OF.MeshInfo3D(5, 6, 6, 4, 5, 5), 2, 1)
+    assert np.array_equal(h(pm.forwardMatrixField(g_(g["q"], dev))), g["prolong_mf"])
+
+
+def g_(a, dev):
+    return g(a, dev)

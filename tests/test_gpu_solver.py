@@ -158,3 +158,15 @@ def test_sweep_tilings_bit_exact(O, dev, sweep_shape):
     uo, po = oracle_solve(O, I0, I1, 2, 2, 6, aniso=False)
     assert np.array_equal(u.cpu().numpy(), uo)
     assert np.array_equal(p.cpu().numpy(), po)
+
+
+def test_gpu_reproduces_golden_solve(dev):
+    """HIP solve of the committed fixture pair equals the oracle's FP32 bits."""
+    import os
+    g = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", "solve_10x11x12.npz"), allow_pickle=False))
+    params = g["params"]
+    solver = make_solver(int(params[0]), int(params[1]), int(params[2]))
+    u, p = solver.compute(torch.from_numpy(g["I0"]).to(dev), torch.from_numpy(g["I1"]).to(dev), None, None)
+    assert np.array_equal(u.cpu().numpy(), g["u_f32"]) and np.array_equal(p.cpu().numpy(), g["p_f32"])
+    err64 = np.abs(u.cpu().numpy().astype(np.float64) - g["u_f64"]).max()
+    print(f"golden: GPU == FP32 oracle; max |u - u_fp64| = {err64:.2e}")
