@@ -1,0 +1,303 @@
+"""Continuous-batching service for flow and mask-propagation requests.
+
+Requests arrive over time; each is a single frame pair or a whole sequence
+(forward + backward flows of every consecutive pair, then propagation of the
+first mask forward and of the last annotated mask backward).  The server
+turns pending requests into work units and admits a unit as soon as a
+stream slot and enough of the HBM budget are free, so work of requests that
+arrived while others were on the GPU overlaps with theirs:
+
+* pair unit: up to ``pair_bucket`` pairs of one volume size, solved as one
+  batched plan (hipGraph replay);
+* sequence unit: the 2(T-1) solves of one sequence as one plan, followed by
+  the tricubic mask propagation on the same stream.
+
+Plans are cached per (volume size, bucket) with their workspaces resident in
+HBM; an idle plan is evicted (LRU) when admission needs its memory.  Every
+instance's arithmetic is independent of the batch it runs in, so a result is
+bit-identical alone and in the mix.
+"""
+import time
+from collections import deque
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from .solver import Plan, SolverConfig, workspace_bytes
+from .warping import Warp, propagate
+
+Shape = Tuple[int, int, int]
+
+
+@dataclass
+class Request:
+    rid: int
+    kind: str                      # "pair" | "sequence"
+    shape: Shape
+    arrival: float                 # seconds on the server clock
+    frames: Sequence[torch.Tensor]  # device volumes; a pair is (I0, I1)
+    masks: Optional[Tuple[torch.Tensor, torch.Tensor, int]] = None  # (mask_first, mask_last, index_last)
+    # results
+    flows: Optional[torch.Tensor] = None          # pair [Z,Y,X,3]; sequence [2(T-1),Z,Y,X,3]
+    propagated: Optional[Tuple[torch.Tensor, torch.Tensor]] = None
+    done: Optional[float] = None
+
+    @property
+    def solves(self):
+        return 1 if self.kind == "pair" else 2 * (len(self.frames) - 1)
+
+
+@dataclass
+class _Unit:
+    requests: List[Request]
+    plan: Plan
+    key: tuple
+    stream: torch.cuda.Stream
+    event: torch.cuda.Event
+    outputs: torch.Tensor
+    nbytes: int
+    slot: int
+
+
+def _bucket(n, buckets):
+    for b in buckets:
+        if n <= b:
+            return b
+    return buckets[-1]
+
+
+class FlowServer:
+    def __init__(self, cfg: SolverConfig, device=None, hbm_budget_bytes: int = 64 << 30, slots: int = 4,
+                 pair_buckets=(1, 2, 4, 8, 16), use_graph: bool = True):
+        self.cfg = cfg
+        dev = torch.device(device) if device is not None else torch.device("cuda")
+        if dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
+        self.budget = int(hbm_budget_bytes)
+        self.pair_buckets = tuple(pair_buckets)
+        self.use_graph = use_graph
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(slots)]
+        self.free_slots = deque(range(slots))
+        self.pending: deque = deque()
+        self.running: List[_Unit] = []
+        self.idle_plans: List[Tuple[tuple, Plan]] = []   # least recently used first
+        self._ws_cache: Dict[tuple, int] = {}
+        self.used = 0            # bytes held by resident plans and in-flight outputs
+        self.warper = Warp("CUBIC_HERMITESPLINE", "MIRROR", "numDofs", (1, 1, 1))
+        self.completed: List[Request] = []
+
+    # ------------------------------------------------------------ memory
+    def _ws(self, key):
+        if key not in self._ws_cache:
+            self._ws_cache[key] = workspace_bytes(self.cfg, [key[0]] * key[1])
+        return self._ws_cache[key]
+
+    def _take_idle(self, key):
+        for i, (k, p) in enumerate(self.idle_plans):
+            if k == key:
+                del self.idle_plans[i]
+                return p
+        return None
+
+    def _plan_for(self, shape: Shape, count: int) -> Tuple[Plan, tuple]:
+        key = (shape, count)
+        plan = self._take_idle(key)
+        if plan is None:
+            self._make_room(self._ws(key))
+            plan = Plan(self.cfg, [shape] * count, device=self.device)
+            self.used += plan.workspace_bytes
+        return plan, key
+
+    def _make_room(self, need):
+        while self.used + need > self.budget and self.idle_plans:
+            _, victim = self.idle_plans.pop(0)
+            self.used -= victim.workspace_bytes
+            victim.close()
+
+    def _fits(self, shape, count, out_bytes):
+        key = (shape, count)
+        have = any(k == key for k, _ in self.idle_plans)
+        need = (0 if have else self._ws(key)) + out_bytes
+        evictable = sum(p.workspace_bytes for k, p in self.idle_plans)
+        if have:
+            evictable -= next(p.workspace_bytes for k, p in self.idle_plans if k == key)
+        return self.used - evictable + need <= self.budget
+
+    # ---------------------------------------------------------- requests
+    def submit(self, req: Request):
+        self.pending.append(req)
+
+    def _next_unit(self) -> Optional[Tuple[List[Request], Shape, int]]:
+        if not self.pending:
+            return None
+        head = self.pending[0]
+        if head.kind == "sequence":
+            return [head], head.shape, head.solves
+        group = [r for r in self.pending if r.kind == "pair" and r.shape == head.shape][:self.pair_buckets[-1]]
+        return group, head.shape, _bucket(len(group), self.pair_buckets)
+
+    def _admit(self, now: float) -> bool:
+        if not self.free_slots:
+            return False
+        nxt = self._next_unit()
+        if nxt is None:
+            return False
+        reqs, shape, count = nxt
+        nvox = shape[0] * shape[1] * shape[2]
+        out_bytes = count * nvox * 12 + (2 * nvox * 4 * 13 if reqs[0].kind == "sequence" else 0)
+        if not self._fits(shape, count, out_bytes):
+            return False
+        for r in reqs:
+            self.pending.remove(r)
+        slot = self.free_slots.popleft()
+        stream = self.streams[slot]
+        plan, key = self._plan_for(shape, count)
+        self.used += out_bytes
+        with torch.cuda.stream(stream):
+            outputs = torch.empty((count,) + shape + (3,), dtype=torch.float32, device=self.device)
+            k = 0
+            for r in reqs:
+                fr = r.frames
+                if r.kind == "pair":
+                    plan.bind(k, fr[0], fr[1], u_out=outputs[k])
+                    k += 1
+                else:
+                    T = len(fr)
+                    for t in range(T - 1):
+                        plan.bind(t, fr[t + 1], fr[t], u_out=outputs[t])
+                        plan.bind(T - 1 + t, fr[t], fr[t + 1], u_out=outputs[T - 1 + t])
+                    k += 2 * (T - 1)
+            while k < count:  # pad the bucket with the last bound pair
+                plan.bind(k, reqs[-1].frames[0], reqs[-1].frames[1], u_out=outputs[k])
+                k += 1
+            plan.run(use_graph=self.use_graph)
+            if reqs[0].kind == "sequence":
+                r = reqs[0]
+                T = len(r.frames)
+                m_first, m_last, j = r.masks
+                fwd = outputs[0:j]
+                bwd = outputs[T - 1:T - 1 + j].flip(0)
+                r.propagated = propagate(self.warper, m_first, m_last, fwd, bwd)
+            event = torch.cuda.Event()
+            event.record(stream)
+        self.running.append(_Unit(reqs, plan, key, stream, event, outputs, out_bytes, slot))
+        return True
+
+    def _retire(self, now: float):
+        still = []
+        for u in self.running:
+            if not u.event.query():
+                still.append(u)
+                continue
+            k = 0
+            for r in u.requests:
+                if r.kind == "pair":
+                    r.flows = u.outputs[k]
+                    k += 1
+                else:
+                    r.flows = u.outputs[:r.solves]
+                    k += r.solves
+                r.done = now
+                self.completed.append(r)
+            self.used -= u.nbytes
+            self.idle_plans.append((u.key, u.plan))   # idle, most recently used
+            self.free_slots.append(u.slot)
+        self.running = still
+
+    def step(self, now: float):
+        self._retire(now)
+        while self._admit(now):
+            pass
+
+    def idle(self):
+        return not self.pending and not self.running
+
+    def close(self):
+        torch.cuda.synchronize(self.device)
+        for _, p in self.idle_plans:
+            p.close()
+        self.idle_plans.clear()
+
+
+# ------------------------------------------------------------ workloads
+def make_pool(shapes: Sequence[Shape], frames: int, device, seed=0):
+    """A resident pool of synthetic sequences per volume size (request inputs)."""
+    from . import phantom
+    pool = {}
+    for i, sh in enumerate(sorted(set(shapes))):
+        seq = torch.from_numpy(phantom.sequence(sh, frames, seed_base=seed + 100 * i)).to(device)
+        masks = phantom.sequence_masks(sh, frames, which=(0, min(12, frames - 1)))
+        m = {t: torch.from_numpy(v.astype(np.float32)).to(device)[None, None] for t, v in masks.items()}
+        pool[sh] = (seq, m)
+    return pool
+
+
+def poisson_requests(rate, count, shapes, pool, frames, seed=4, p_pair=0.7, mask_last=12):
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        sh = shapes[int(rng.integers(0, len(shapes)))]
+        seq, m = pool[sh]
+        if rng.random() < p_pair:
+            j = int(rng.integers(0, frames - 1))
+            out.append(Request(i, "pair", sh, t, (seq[j + 1], seq[j])))
+        else:
+            jl = min(mask_last, frames - 1)
+            out.append(Request(i, "sequence", sh, t, [seq[k] for k in range(frames)], masks=(m[0], m[jl], jl)))
+    return out
+
+
+def run_service(server: FlowServer, requests: List[Request], poll_s: float = 2e-4):
+    """Feeds requests at their arrival times (wall clock) until all completed.
+
+    Returns (latencies in s, makespan in s)."""
+    arrivals = sorted(requests, key=lambda r: r.arrival)
+    torch.cuda.synchronize(server.device)
+    t0 = time.perf_counter()
+    i = 0
+    while i < len(arrivals) or not server.idle():
+        now = time.perf_counter() - t0
+        while i < len(arrivals) and arrivals[i].arrival <= now:
+            server.submit(arrivals[i])
+            i += 1
+        server.step(now)
+        if not server.running and i < len(arrivals) and not server.pending:
+            wait = arrivals[i].arrival - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(min(wait, 0.01))
+        else:
+            time.sleep(poll_s)
+    torch.cuda.synchronize(server.device)
+    makespan = time.perf_counter() - t0
+    lat = np.array([r.done - r.arrival for r in requests])
+    return lat, makespan
+
+
+def run_plain(solver, requests: List[Request], warper=None):
+    """The plain per-request drop-in: one pair at a time, in arrival order."""
+    warper = warper or Warp("CUBIC_HERMITESPLINE", "MIRROR", "numDofs", (1, 1, 1))
+    arrivals = sorted(requests, key=lambda r: r.arrival)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in arrivals:
+        now = time.perf_counter() - t0
+        if r.arrival > now:
+            time.sleep(r.arrival - now)
+        if r.kind == "pair":
+            r.flows, _ = solver.compute(r.frames[0], r.frames[1], None, None)
+        else:
+            T = len(r.frames)
+            fl = [solver.compute(r.frames[t + 1], r.frames[t], None, None)[0] for t in range(T - 1)]
+            fl += [solver.compute(r.frames[t], r.frames[t + 1], None, None)[0] for t in range(T - 1)]
+            r.flows = torch.stack(fl)
+            m_first, m_last, j = r.masks
+            r.propagated = propagate(warper, m_first, m_last, r.flows[0:j], r.flows[T - 1:T - 1 + j].flip(0))
+        torch.cuda.synchronize()
+        r.done = time.perf_counter() - t0
+    makespan = time.perf_counter() - t0
+    return np.array([r.done - r.arrival for r in requests]), makespan

@@ -89,6 +89,13 @@ BYTES_PER_VOXEL_ITERATION = 124  # u 12 + p 36 + rho_c 4 + g 12 + grad I1 12 rea
 BYTES_PER_VOXEL_WARP = 48        # u 12, I0 4, I1 4, grad I1 12 read; rho_c 4, g 12 written
 
 
+def workspace_bytes(cfg: SolverConfig, shapes) -> int:
+    """Device workspace a plan for these instance sizes needs (0 if invalid)."""
+    params, _w = cfg.native()
+    dims = (ctypes.c_int * (3 * len(shapes)))(*[int(v) for s in shapes for v in s])
+    return int(_lib.lib().svs_tvl1_workspace_bytes(ctypes.byref(params), len(shapes), dims))
+
+
 def algorithmic_bytes(cfg: SolverConfig, shape):
     return BYTES_PER_VOXEL_ITERATION * cfg.voxel_iterations(shape) + BYTES_PER_VOXEL_WARP * cfg.voxel_warps(shape)
 

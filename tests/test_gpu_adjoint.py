@@ -1,0 +1,91 @@
+"""The hand-written tricubic warp adjoint (volume and flow) behind
+WarpingOpFunction: against autograd through a plain-PyTorch Catmull-Rom warp
+and against FP64 central finite differences of the CPU oracle on a 16^3 crop."""
+import numpy as np
+import pytest
+import torch
+
+from singleventriclesegmentation_amd import opticalFlow as OF
+from singleventriclesegmentation_amd.warpingOps import warp
+from torch_warp_ref import warp_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def cnn_op(shape):
+    nz, ny, nx = shape
+    return OF.WarpingCNN3D(OF.MeshInfo3D(nz, ny, nx, nz - 1, ny - 1, nx - 1), 2, 1)
+
+
+def test_forward_matches_torch_reference(dev):
+    r = np.random.default_rng(0)
+    shape = (16, 16, 16)
+    u = torch.from_numpy(r.standard_normal((2, 3) + shape)).to(dev, torch.float32)
+    phi = torch.from_numpy(r.uniform(-2, 2, (2,) + shape + (3,))).to(dev, torch.float32)
+    ours = cnn_op(shape).forward(u, phi)
+    ref = warp_ref(u.double().cpu(), phi.double().cpu())
+    assert (ours.double().cpu() - ref).abs().max().item() < 1e-5
+
+
+def test_adjoint_matches_autograd_of_torch_reference(dev):
+    r = np.random.default_rng(1)
+    shape = (16, 16, 16)
+    u = torch.from_numpy(r.standard_normal((2, 2) + shape)).to(dev, torch.float32).requires_grad_(True)
+    phi = torch.from_numpy(r.uniform(-2, 2, (2,) + shape + (3,))).to(dev, torch.float32).requires_grad_(True)
+    g = torch.from_numpy(r.standard_normal((2, 2) + shape)).to(dev, torch.float32)
+    (warp(u, phi, cnn_op(shape)) * g).sum().backward()
+    u64 = u.detach().double().cpu().requires_grad_(True)
+    phi64 = phi.detach().double().cpu().requires_grad_(True)
+    (warp_ref(u64, phi64) * g.double().cpu()).sum().backward()
+    gu_err = (u.grad.double().cpu() - u64.grad).abs().max().item() / u64.grad.abs().max().item()
+    gp_err = (phi.grad.double().cpu() - phi64.grad).abs().max().item() / phi64.grad.abs().max().item()
+    print(f"adjoint vs autograd: grad_u rel {gu_err:.2e}, grad_phi rel {gp_err:.2e}")
+    assert gu_err < 1e-5 and gp_err < 1e-5
+
+
+def test_adjoint_matches_fp64_finite_differences(O, dev):
+    """16^3 crop of a phantom, loss = sum(g * warp(u, phi)); central FD in FP64 (oracle)."""
+    from singleventriclesegmentation_amd import phantom
+    shape = (16, 16, 16)
+    vol = phantom.clean_frame((32, 32, 32))[8:24, 8:24, 8:24].astype(np.float64)
+    r = np.random.default_rng(2)
+    phi = phantom.displacement((32, 32, 32), peak=2.0)[8:24, 8:24, 8:24].astype(np.float64)
+    phi = phi + r.uniform(-0.3, 0.3, phi.shape)
+    g = r.standard_normal(shape)
+    op = OF.Warping3D(OF.MeshInfo3D(16, 16, 16, 15, 15, 15), 2, 1)
+    gu, gphi = op.backward(torch.from_numpy(vol).float().to(dev), torch.from_numpy(phi).float().to(dev),
+                           torch.from_numpy(g).float().to(dev))
+    gu, gphi = gu.cpu().numpy(), gphi.cpu().numpy()
+    eps = 1e-6
+
+    def loss(u_, p_):
+        return float(np.sum(g * O.warp(u_, p_, dtype=np.float64)))
+
+    worst = 0.0
+    for k in range(30):
+        idx = tuple(int(v) for v in r.integers(0, 16, 3))
+        c = k % 3
+        pp, pm = phi.copy(), phi.copy()
+        pp[idx + (c,)] += eps
+        pm[idx + (c,)] -= eps
+        fd = (loss(vol, pp) - loss(vol, pm)) / (2 * eps)
+        worst = max(worst, abs(fd - gphi[idx + (c,)]) / max(1e-3, abs(fd)))
+        up, um = vol.copy(), vol.copy()
+        up[idx] += eps
+        um[idx] -= eps
+        fd_u = (loss(up, phi) - loss(um, phi)) / (2 * eps)
+        worst = max(worst, abs(fd_u - gu[idx]) / max(1e-3, abs(fd_u)))
+    print(f"adjoint vs FP64 central differences: worst relative error {worst:.2e}")
+    assert worst < 2e-4
+
+
+def test_refinement_step_runs_and_learns(dev):
+    from singleventriclesegmentation_amd.refinement import Refiner, RefinementConfig, synthetic_batch
+    shape = (24, 24, 24)
+    batch = synthetic_batch(shape, 2, dev)
+    ref = Refiner(shape, 2, dev, RefinementConfig(features_start=4))
+    first = ref.step(*batch)["total"].item()
+    for _ in range(15):
+        last = ref.step(*batch)["total"].item()
+    assert np.isfinite(last) and last < first
+    assert ref.flow_res.grad is not None and ref.flow_res.grad.abs().sum().item() > 0
