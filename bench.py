@@ -36,6 +36,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference kernel for PMC byte counters)")
+    ap.add_argument("--workload", default="sequence", choices=["sequence", "uneven", "serve", "unet"],
+                    help="sequence: 35-frame batch (the headline line); uneven: 96 pairs of mixed sizes; "
+                         "serve: Poisson request service; unet: refinement training with the warp adjoint")
+    ap.add_argument("--pairs", type=int, default=96, help="uneven: number of pairs")
+    ap.add_argument("--requests", type=int, default=40, help="serve: requests per arrival rate")
+    ap.add_argument("--rates", default="5,20,50", help="serve: arrival rates (req/s)")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -63,6 +69,8 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.workload != "sequence":
+        return OTHER[args.workload](args, dev)
     shape = (args.size,) * 3
     T = args.frames
     cfg = SolverConfig(num_scales=4, max_warps=10, max_outer_iterations=50)
@@ -88,9 +96,12 @@ def main():
         bwd = u_all[T - 1:T - 1 + 12].flip(0)  # flows 12->11 ... 1->0
         return propagate(warper, m0, m12, fwd, bwd)
 
-    log(args, "warm-up (graph capture on first run)")
+    log(args, "warm-up and sweep-shape autotuning")
+    step()
+    tuned = plan.autotune()
+    log(args, "tuned sweep shapes (bx, by, cz) per level:", tuned)
     for _ in range(args.warmup):
-        step()
+        step()  # the first step after tuning re-captures the graph
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -126,9 +137,10 @@ def main():
 
     # Plain per-pair drop-in (TVL13DOpticalFlow.compute, one pair at a time).
     plain = TVL13DOpticalFlow(4, 10, 50, 1.0, 0.025, 2, (0.5, 0.5, 0.25, 1.0), True, (5.0, 0.5), True, 1.0,
-                              "CUBIC_HERMITESPLINE", "MIRROR", "numDofs", (1, 1, 1), dev)
+                              "CUBIC_HERMITESPLINE", "MIRROR", "numDofs", (1, 1, 1), dev, autotune=True)
     plain_sps = None
     if args.plain_pairs > 0:
+        plain.compute(seq[1], seq[0], None, None)  # first call also autotunes the single-pair plan
         plain.compute(seq[1], seq[0], None, None)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -180,7 +192,8 @@ def main():
         "config": {"workload": f"full-sequence batch: {T} frames of {shape[0]}^3, {T - 1} forward + {T - 1} "
                                f"backward pair solves in one batched hipGraph plan + mask propagation 0<->12",
                    "solves_per_step": npairs, "levels": cfg.num_scales, "warps": cfg.max_warps,
-                   "iterations": cfg.max_outer_iterations, "shape_zyx": list(shape)},
+                   "iterations": cfg.max_outer_iterations, "shape_zyx": list(shape),
+                   "sweep_shapes_bx_by_cz": [list(t) for t in tuned]},
         "plain_solves_per_s": round(plain_sps, 3) if plain_sps else None,
         "batched_over_plain": round(solves_per_s / plain_sps, 3) if plain_sps else None,
         "voxel_iterations_per_s": round(solves_per_s * vox_it, 1),
@@ -194,6 +207,132 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def _line(args, metric, value, unit, ms_per_step, config, extra, higher=True):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms_per_step, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic phantoms", "config": config}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def bench_uneven(args, dev):
+    """96 frame pairs of sizes drawn from the five-size mix, one batched plan."""
+    import numpy as np
+    import torch
+    from singleventriclesegmentation_amd import opticalFlow as OF
+    from singleventriclesegmentation_amd import phantom
+    from singleventriclesegmentation_amd.solver import (BYTES_PER_VOXEL_ITERATION, Plan, SolverConfig,
+                                                        algorithmic_bytes)
+    cfg = SolverConfig(num_scales=4, max_warps=10, max_outer_iterations=50)
+    shapes = phantom.uneven_shapes(args.pairs, seed=3)
+    frames = {}
+    for sh in sorted(set(shapes)):
+        clean = phantom.clean_frame(sh)
+        d = torch.from_numpy(phantom.displacement(sh, peak=3.0)).to(dev)
+        op = OF.WarpingCNN3D(OF.MeshInfo3D(*sh, sh[0] - 1, sh[1] - 1, sh[2] - 1), 2, 1)
+        moved = op.forward(torch.from_numpy(clean).to(dev)[None, None], d[None].contiguous())[0, 0]
+        f0 = torch.from_numpy(phantom.add_noise(clean, 0)).to(dev)
+        f1 = (moved + torch.from_numpy(np.random.default_rng(1).normal(0, 0.02, sh).astype(np.float32)).to(dev))
+        frames[sh] = (f1.clamp(0, 1).contiguous(), f0)
+    plan = Plan(cfg, shapes, device=dev)
+    outs = [torch.empty(sh + (3,), device=dev) for sh in shapes]
+    for k, sh in enumerate(shapes):
+        plan.bind(k, frames[sh][0], frames[sh][1], u_out=outs[k])
+    plan.run(use_graph=True)
+    tuned = plan.autotune()
+    for _ in range(args.warmup):
+        plan.run(use_graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.run(use_graph=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    vox_it = sum(cfg.voxel_iterations(sh) for sh in shapes)
+    abytes = sum(algorithmic_bytes(cfg, sh) for sh in shapes)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    plan.sweeps(0, args.sweep_launches)
+    e1.record()
+    e1.synchronize()
+    l0_us = e0.elapsed_time(e1) * 1e3 / args.sweep_launches
+    l0_bytes = BYTES_PER_VOXEL_ITERATION * sum(a * b * c for a, b, c in shapes)
+    _line(args, "uneven-size batch: TV-L1 solves/s", round(len(shapes) * args.steps / el, 3), "solves/s",
+          round(1e3 * el / args.steps, 3),
+          {"workload": f"{len(shapes)} pairs, sizes from {{48^3, 64x64x40, 80^3, 96x96x64, 128x128x48}}, 4x10x50",
+           "sizes_zyx": sorted(set(shapes)), "sweep_shapes_bx_by_cz": [list(t) for t in tuned]},
+          {"voxel_iterations_per_s": round(vox_it * args.steps / el, 1),
+           "algorithmic_GBps": round(abytes * args.steps / el / 1e9, 1),
+           "level0_sweep_GBps": round(l0_bytes / (l0_us * 1e-6) / 1e9, 1), "level0_sweep_us": round(l0_us, 1)})
+
+
+def bench_serve(args, dev):
+    """Poisson arrivals (70% pairs, 30% 35-frame sequences) against a 64 GB budget."""
+    import numpy as np
+    import torch
+    from singleventriclesegmentation_amd import TVL13DOpticalFlow, phantom
+    from singleventriclesegmentation_amd.serve import FlowServer, make_pool, poisson_requests, run_plain, run_service
+    from singleventriclesegmentation_amd.solver import SolverConfig
+    cfg = SolverConfig(num_scales=4, max_warps=10, max_outer_iterations=50)
+    sizes = list(phantom.UNEVEN_SIZES)
+    log(args, "building request pool")
+    pool = make_pool(sizes, args.frames, dev)
+    server = FlowServer(cfg, device=dev, hbm_budget_bytes=64 << 30, slots=4)
+    warm = poisson_requests(0, 6, sizes, pool, args.frames, seed=99)
+    run_service(server, warm)
+    res = {}
+    total_solves, total_time = 0, 0.0
+    for rate in [float(r) for r in args.rates.split(",")]:
+        reqs = poisson_requests(rate, args.requests, sizes, pool, args.frames, seed=4)
+        lat, span = run_service(server, reqs)
+        solves = sum(r.solves for r in reqs)
+        total_solves += solves
+        total_time += span
+        res[f"{rate:g}"] = {"p50_s": round(float(np.percentile(lat, 50)), 3),
+                            "p99_s": round(float(np.percentile(lat, 99)), 3),
+                            "solves_per_s": round(solves / span, 2), "requests": len(reqs), "solves": solves}
+        log(args, f"rate {rate}: {res[f'{rate:g}']}")
+    server.close()
+    plain = TVL13DOpticalFlow(4, 10, 50, 1.0, 0.025, 2, (0.5, 0.5, 0.25, 1.0), True, (5.0, 0.5), True, 1.0,
+                              "CUBIC_HERMITESPLINE", "MIRROR", "numDofs", (1, 1, 1), dev)
+    reqs = poisson_requests(5.0, max(4, args.requests // 8), sizes, pool, args.frames, seed=4)
+    lat, span = run_plain(plain, reqs)
+    plain_res = {"rate": 5.0, "requests": len(reqs), "p50_s": round(float(np.percentile(lat, 50)), 3),
+                 "p99_s": round(float(np.percentile(lat, 99)), 3),
+                 "solves_per_s": round(sum(r.solves for r in reqs) / span, 2)}
+    _line(args, "served TV-L1 flow: solves/s with p50/p99 latency", round(total_solves / total_time, 3), "solves/s",
+          None, {"workload": f"Poisson arrivals at {args.rates} req/s, {args.requests} requests each; 70% pairs, "
+                             f"30% {args.frames}-frame fwd+bwd sequences with mask propagation; sizes from the "
+                             "five-size mix; 64 GB budget; 4 stream slots"},
+          {"by_rate": res, "plain_drop_in": plain_res})
+
+
+def bench_unet(args, dev):
+    """Refinement training: batch 4 of 80^3, U-Net 4 levels / 16 base channels, warp adjoint in the loss."""
+    import torch
+    from singleventriclesegmentation_amd.refinement import Refiner, RefinementConfig, synthetic_batch
+    from singleventriclesegmentation_amd.unet import parameter_count
+    shape = (args.size,) * 3
+    batch = synthetic_batch(shape, 4, dev)
+    ref = Refiner(shape, 4, dev, RefinementConfig())
+    for _ in range(args.warmup):
+        ref.step(*batch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = ref.step(*batch)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    _line(args, "U-Net refinement training steps/s (tricubic warp + adjoint in the loss)",
+          round(args.steps / el, 3), "steps/s", round(1e3 * el / args.steps, 3),
+          {"workload": f"batch 4 x 2 channels x {shape[0]}^3, 4-class softmax, consistency loss through the warp",
+           "params": parameter_count(ref.model)},
+          {"loss": round(float(out["total"].item()), 4)})
+
+
+OTHER = {"uneven": bench_uneven, "serve": bench_serve, "unet": bench_unet}
 
 
 if __name__ == "__main__":

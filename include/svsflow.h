@@ -171,6 +171,14 @@ int svs_tvl1_plan_info(const svs_plan* plan, int level, int* out);
  * its own with events around it).  No reference counterpart. */
 int svs_tvl1_plan_sweeps(svs_plan* plan, int level, int count, void* stream);
 
+/* Autotuner: times candidate fused-sweep shapes (work-group tile and
+ * z-chunk) per level on the plan's state (needs one prior run), keeps the
+ * fastest, remembers it for later plans of the same geometry in this
+ * process and drops the captured graph.  report (optional) receives
+ * bx, by, cz per level.  Results do not depend on the shape.  No reference
+ * counterpart. */
+int svs_tvl1_plan_autotune(svs_plan* plan, int reps, void* stream, int* report);
+
 /* One frame pair without a graph: TVL13DOpticalFlow.compute(I0, I1, u, p). */
 int svs_tvl1_solve(const svs_tvl1_params* params, const float* I0, const float* I1, int NZ, int NY, int NX,
                    const float* u_in, const float* p_in, float* u_out, float* p_out, void* workspace,

@@ -16,9 +16,13 @@
 //   p[2]       float[9][N]  SoA    dual, p[3c+d] = row c, derivative d
 // J, SN and RG are shared by the levels of an instance (levels run in turn).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "svs_common.hpp"
@@ -476,6 +480,7 @@ struct LevelPlan {
   int nblocks = 0;
   SweepTile* tiles = nullptr;
   int ntiles = 0;
+  int tile_capacity = 0;
   int bx = 0, by = 0, cz = 0;
 };
 
@@ -514,6 +519,84 @@ static void level_geometry(const svs_tvl1_params& P, int NZ, int NY, int NX, Lev
   d.N = (int64_t)NZ * NY * NX;
 }
 
+static size_t sweep_lds_bytes(int bx, int by) {
+  return sizeof(float) * (6 * (size_t)(bx + 2) * (by + 2) + 12 * (size_t)bx * by);
+}
+
+struct SweepShape {
+  int bx, by, cz;
+};
+
+static int count_tiles(const LevelPlan& L, int bx, int by, int cz) {
+  const int tx = bx - 2, ty = by - 2;
+  int n = 0;
+  for (auto& d : L.host) n += ((d.NX + tx - 1) / tx) * ((d.NY + ty - 1) / ty) * ((d.NZ + cz - 1) / cz);
+  return n;
+}
+
+static bool shape_ok(int bx, int by, int cz) {
+  return bx >= 3 && by >= 3 && cz >= 1 && bx * by <= 1024 && 2 * (bx + 2) + 2 * by <= bx * by &&
+         sweep_lds_bytes(bx, by) <= 96 * 1024;
+}
+
+// z-chunk that gives the launch about `target` work-groups.
+static int default_cz(const LevelPlan& L, int bx, int by, int target, int minz) {
+  int maxNZ = 0;
+  for (auto& d : L.host) maxNZ = std::max(maxNZ, d.NZ);
+  const int tx = bx - 2, ty = by - 2;
+  int64_t cols = 0;
+  for (auto& d : L.host) cols += (int64_t)((d.NX + tx - 1) / tx) * ((d.NY + ty - 1) / ty);
+  int cz = maxNZ;
+  while (cz > minz && cols * ((maxNZ + cz - 1) / cz) < target) cz = (cz + 1) / 2;
+  return cz;
+}
+
+// Autotuner candidates: (tile width, tile height) pairs with a z-chunk aimed at
+// ~2048 work-groups (stage 1), then other z-chunks for the winner (stage 2).
+static std::vector<SweepShape> tile_candidates(const LevelPlan& L) {
+  int maxNX = 0, maxNY = 0;
+  for (auto& d : L.host) { maxNX = std::max(maxNX, d.NX); maxNY = std::max(maxNY, d.NY); }
+  std::vector<int> txs = {maxNX, (maxNX + 1) / 2, (maxNX + 2) / 3, (maxNX + 3) / 4, 16, 8};
+  std::vector<int> tys = {2, 4, 6, 8, 10, 14, 20};
+  std::vector<SweepShape> out;
+  for (int tx : txs) {
+    if (tx < 4 || tx > 126) continue;
+    for (int ty : tys) {
+      if (ty > maxNY && ty != 2) continue;
+      const int bx = tx + 2, by = ty + 2;
+      bool dup = false;
+      for (auto& c : out) dup |= (c.bx == bx && c.by == by);
+      if (dup || !shape_ok(bx, by, 1)) continue;
+      out.push_back({bx, by, default_cz(L, bx, by, 2048, 2)});
+    }
+  }
+  return out;
+}
+
+static std::vector<int> cz_candidates(const LevelPlan& L) {
+  int maxNZ = 0;
+  for (auto& d : L.host) maxNZ = std::max(maxNZ, d.NZ);
+  std::vector<int> out;
+  for (int cz : {1, 2, 4, 8, 16, 40, maxNZ})
+    if (cz <= maxNZ && std::find(out.begin(), out.end(), cz) == out.end()) out.push_back(cz);
+  return out;
+}
+
+// Process-wide memory of tuned shapes, keyed by the level geometry and the
+// kernel variant, so later plans of the same geometry start tuned.
+static std::mutex g_tune_mu;
+static std::map<std::string, SweepShape> g_tuned;
+
+static std::string tune_key(const svs_tvl1_params& P, const LevelPlan& L) {
+  std::map<std::array<int, 3>, int> hist;
+  for (auto& d : L.host) hist[{d.NZ, d.NY, d.NX}]++;
+  std::string k = std::to_string(P.boundary) + "/" + std::to_string(P.use_aniso) + ":";
+  for (auto& e : hist)
+    k += std::to_string(e.first[0]) + "x" + std::to_string(e.first[1]) + "x" + std::to_string(e.first[2]) + "*" +
+         std::to_string(e.second) + ",";
+  return k;
+}
+
 // Sweep launch shape for a level: (BX, BY) threads per work-group (the tile
 // plus its one-voxel ring) and the z-chunk.  Large levels (a batch of many
 // volumes) use full-row tiles 8 rows high marching over long z-chunks; small
@@ -533,6 +616,14 @@ static void choose_sweep_shape(const svs_tvl1_params& P, LevelPlan& L, int ninst
   const bool big = vox >= ((int64_t)8 << 20);
   const bool tiny = vox < ((int64_t)1 << 20);
   int bx = P.sweep_bx, by = P.sweep_by, cz = P.sweep_cz;
+  if (bx <= 0 && by <= 0 && cz <= 0) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(tune_key(P, L));
+    if (it != g_tuned.end()) {
+      L.bx = it->second.bx; L.by = it->second.by; L.cz = it->second.cz;
+      return;
+    }
+  }
   if (bx <= 0) {
     const int maxtx = big ? 64 : (tiny ? 10 : 20);
     const int ntx = (maxNX + maxtx - 1) / maxtx;
@@ -554,10 +645,6 @@ static void choose_sweep_shape(const svs_tvl1_params& P, LevelPlan& L, int ninst
     while (cz > minz && cols * ((maxNZ + cz - 1) / cz) < target) cz = (cz + 1) / 2;
   }
   L.bx = bx; L.by = by; L.cz = cz;
-}
-
-static size_t sweep_lds_bytes(int bx, int by) {
-  return sizeof(float) * (6 * (size_t)(bx + 2) * (by + 2) + 12 * (size_t)bx * by);
 }
 
 }  // namespace svs
@@ -606,12 +693,15 @@ static int plan_layout(svs_plan* pl, char* base, size_t* total_out) {
     choose_sweep_shape(P, L, pl->ninst);
     L.dev = (LevelDesc*)take(sizeof(LevelDesc) * pl->ninst);
     L.block_start = (int*)take(sizeof(int) * (pl->ninst + 1));
-    const int tx = L.bx - 2, ty = L.by - 2;
-    int ntiles = 0;
-    for (auto& d : L.host)
-      ntiles += ((d.NX + tx - 1) / tx) * ((d.NY + ty - 1) / ty) * ((d.NZ + L.cz - 1) / L.cz);
-    L.ntiles = ntiles;
-    L.tiles = (SweepTile*)take(sizeof(SweepTile) * ntiles);
+    L.ntiles = count_tiles(L, L.bx, L.by, L.cz);
+    // Room for any shape the autotuner may try.
+    int cap = L.ntiles;
+    for (const SweepShape& c : tile_candidates(L)) {
+      cap = std::max(cap, count_tiles(L, c.bx, c.by, c.cz));
+      for (int cz : cz_candidates(L)) cap = std::max(cap, count_tiles(L, c.bx, c.by, cz));
+    }
+    L.tile_capacity = cap;
+    L.tiles = (SweepTile*)take(sizeof(SweepTile) * cap);
     int nb = 0;
     for (auto& d : L.host) nb += grid_for(d.N, kBlock);
     L.nblocks = nb;
@@ -685,6 +775,8 @@ static int upload_tables(svs_plan* pl, hipStream_t st) {
   return 0;
 }
 
+static int upload_tiles(svs_plan* pl, int s, hipStream_t st);
+
 static int upload_static(svs_plan* pl, hipStream_t st) {
   for (int s = 0; s < pl->S; ++s) {
     LevelPlan& L = pl->levels[s];
@@ -692,6 +784,24 @@ static int upload_static(svs_plan* pl, hipStream_t st) {
     int acc = 0;
     for (int k = 0; k < pl->ninst; ++k) { bs[k] = acc; acc += grid_for(L.host[k].N, kBlock); }
     bs[pl->ninst] = acc;
+    SVS_HIP(hipMemcpyAsync(L.block_start, bs.data(), sizeof(int) * bs.size(), hipMemcpyHostToDevice, st));
+    SVS_HIP(hipStreamSynchronize(st));
+    if (int rc = upload_tiles(pl, s, st)) return rc;
+  }
+  if (pl->P.use_blur) {
+    SVS_HIP(hipMemcpyAsync(pl->blur_dev, pl->blur_w.data(), sizeof(float) * pl->blur_w.size(),
+                           hipMemcpyHostToDevice, st));
+    SVS_HIP(hipStreamSynchronize(st));
+  }
+  return 0;
+}
+
+// Builds and uploads level s's tile list for its current (bx, by, cz).
+static int upload_tiles(svs_plan* pl, int s, hipStream_t st) {
+  {
+    LevelPlan& L = pl->levels[s];
+    L.ntiles = count_tiles(L, L.bx, L.by, L.cz);
+    SVS_REQUIRE(L.ntiles <= L.tile_capacity, "tile table overflow (%d > %d)", L.ntiles, L.tile_capacity);
     std::vector<SweepTile> natural;
     natural.reserve(L.ntiles);
     const int tx = L.bx - 2, ty = L.by - 2;
@@ -718,13 +828,7 @@ static int upload_static(svs_plan* pl, hipStream_t st) {
         for (int g = 0; g < nx; ++g)
           if (start[g] + slot < start[g + 1]) tiles.push_back(natural[start[g] + slot]);
     }
-    SVS_HIP(hipMemcpyAsync(L.block_start, bs.data(), sizeof(int) * bs.size(), hipMemcpyHostToDevice, st));
     SVS_HIP(hipMemcpyAsync(L.tiles, tiles.data(), sizeof(SweepTile) * tiles.size(), hipMemcpyHostToDevice, st));
-    SVS_HIP(hipStreamSynchronize(st));
-  }
-  if (pl->P.use_blur) {
-    SVS_HIP(hipMemcpyAsync(pl->blur_dev, pl->blur_w.data(), sizeof(float) * pl->blur_w.size(),
-                           hipMemcpyHostToDevice, st));
     SVS_HIP(hipStreamSynchronize(st));
   }
   return 0;
@@ -756,9 +860,9 @@ static void launch_sweep(const svs_plan* pl, const LevelPlan& L, int parity, int
 
 // Dynamic LDS above 64 KiB must be enabled per kernel.
 static int allow_large_lds(const svs_plan* pl) {
-  size_t need = 0;
+  // Autotuner candidates stay within 96 KiB; enable that much once.
+  size_t need = 96 * 1024;
   for (const LevelPlan& L : pl->levels) need = std::max(need, sweep_lds_bytes(L.bx, L.by));
-  if (need <= 64 * 1024) return 0;
   const int v = (int)need;
   SVS_HIP(hipFuncSetAttribute((const void*)k_sweep<BD_NEAREST, false>, hipFuncAttributeMaxDynamicSharedMemorySize, v));
   SVS_HIP(hipFuncSetAttribute((const void*)k_sweep<BD_NEAREST, true>, hipFuncAttributeMaxDynamicSharedMemorySize, v));
@@ -947,6 +1051,62 @@ int svs_tvl1_plan_sweeps(svs_plan* pl, int level, int count, void* stream) {
     parity ^= 1;
   }
   return 0;
+}
+
+// Times candidate sweep shapes per level on the plan's own state and keeps
+// the fastest (stage 1: tile shape with a default z-chunk; stage 2: z-chunk).
+// Results are remembered for later plans of the same geometry.  Drops the
+// captured graph, which the next graph run re-captures.
+int svs_tvl1_plan_autotune(svs_plan* pl, int reps, void* stream, int* report /* 3 per level, may be NULL */) {
+  SVS_REQUIRE(pl && reps >= 1, "bad plan/reps");
+  SVS_REQUIRE(!pl->tables_dirty, "run the plan once before autotuning");
+  SVS_REQUIRE(pl->P.max_outer_iterations > 0, "no iterations configured");
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  SVS_HIP(hipEventCreate(&e0));
+  SVS_HIP(hipEventCreate(&e1));
+  int rc = 0;
+  for (int s = 0; s < pl->S && !rc; ++s) {
+    LevelPlan& L = pl->levels[s];
+    auto time_shape = [&](int bx, int by, int cz, float* ms) -> int {
+      L.bx = bx; L.by = by; L.cz = cz;
+      if (int r = upload_tiles(pl, s, st)) return r;
+      launch_sweep(pl, L, 0, 0, st);
+      SVS_LAUNCHED();
+      SVS_HIP(hipEventRecord(e0, st));
+      for (int i = 0; i < reps; ++i) launch_sweep(pl, L, i & 1, i % pl->P.max_outer_iterations, st);
+      SVS_LAUNCHED();
+      SVS_HIP(hipEventRecord(e1, st));
+      SVS_HIP(hipEventSynchronize(e1));
+      SVS_HIP(hipEventElapsedTime(ms, e0, e1));
+      return 0;
+    };
+    SweepShape best{L.bx, L.by, L.cz};
+    float best_ms = 0.0f;
+    if ((rc = time_shape(best.bx, best.by, best.cz, &best_ms))) break;
+    for (const SweepShape& c : tile_candidates(L)) {
+      float ms = 0.0f;
+      if ((rc = time_shape(c.bx, c.by, c.cz, &ms))) break;
+      if (ms < best_ms) { best_ms = ms; best = c; }
+    }
+    if (rc) break;
+    for (int cz : cz_candidates(L)) {
+      float ms = 0.0f;
+      if ((rc = time_shape(best.bx, best.by, cz, &ms))) break;
+      if (ms < best_ms) { best_ms = ms; best.cz = cz; }
+    }
+    if (rc) break;
+    L.bx = best.bx; L.by = best.by; L.cz = best.cz;
+    if ((rc = upload_tiles(pl, s, st))) break;
+    if (report) { report[3 * s] = best.bx; report[3 * s + 1] = best.by; report[3 * s + 2] = best.cz; }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[tune_key(pl->P, L)] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (pl->exec) { (void)hipGraphExecDestroy(pl->exec); pl->exec = nullptr; }
+  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  return rc;
 }
 
 // One frame pair, enqueued directly (no graph): the plain drop-in.

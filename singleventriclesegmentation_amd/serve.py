@@ -229,7 +229,7 @@ def make_pool(shapes: Sequence[Shape], frames: int, device, seed=0):
     pool = {}
     for i, sh in enumerate(sorted(set(shapes))):
         seq = torch.from_numpy(phantom.sequence(sh, frames, seed_base=seed + 100 * i)).to(device)
-        masks = phantom.sequence_masks(sh, frames, which=(0, min(12, frames - 1)))
+        masks = phantom.sequence_masks(sh, frames, which=tuple(range(frames)))
         m = {t: torch.from_numpy(v.astype(np.float32)).to(device)[None, None] for t, v in masks.items()}
         pool[sh] = (seq, m)
     return pool

@@ -156,6 +156,16 @@ class Plan:
         with torch.cuda.device(self.device):
             check(_lib.lib().svs_tvl1_plan_run(self._h, 1 if use_graph else 0, stream_ptr()))
 
+    def autotune(self, reps=3):
+        """Picks the fastest fused-sweep shape per level (needs one prior run).
+
+        Returns [(bx, by, cz)] per level; later plans of the same geometry in
+        this process start from these shapes."""
+        rep = (ctypes.c_int * (3 * self.cfg.num_scales))()
+        with torch.cuda.device(self.device):
+            check(_lib.lib().svs_tvl1_plan_autotune(self._h, reps, stream_ptr(), rep))
+        return [tuple(rep[3 * i:3 * i + 3]) for i in range(self.cfg.num_scales)]
+
     def sweeps(self, level, count):
         """Enqueues ``count`` fused sweeps of one level (timing aid; needs a prior run)."""
         with torch.cuda.device(self.device):

@@ -37,11 +37,14 @@ class TVL13DOpticalFlow:
         length: Tuple[int, int, int],
         device: torch.device,
         use_graph: bool = True,
+        autotune: bool = False,
     ):
         """Arguments as in tvl13d.py:14-52; ``length`` is (lx, ly, lz).
 
-        ``use_graph`` (additive) replays the fixed level x warp x iteration
-        schedule as a hipGraph."""
+        Additive options: ``use_graph`` replays the fixed level x warp x
+        iteration schedule as a hipGraph; ``autotune`` times the fused-sweep
+        launch shapes after the first solve of each volume size and keeps the
+        fastest (results do not depend on the shape)."""
         self.num_scales = num_scales
         self.max_warps = max_warps
         self.max_outer_iterations = max_outer_iterations
@@ -60,6 +63,7 @@ class TVL13DOpticalFlow:
         self.interpolation_type, self.boundary_type = get_interpolation_type(interpolation_type, boundary_type)
         self.device = device
         self.use_graph = use_graph
+        self.autotune = autotune
         self.config = SolverConfig(
             num_scales=num_scales, max_warps=max_warps, max_outer_iterations=max_outer_iterations,
             weight_matching=weight_matching, weight_tv=weight_tv, pd_type=primaldual_algorithm_type,
@@ -104,6 +108,9 @@ class TVL13DOpticalFlow:
         p_out = torch.empty(shape + (3, 3), dtype=torch.float32, device=I0.device)
         plan.bind(0, I0, I1, u_out=u_out, p_out=p_out, u_in=u_in, p_in=p_in)
         plan.run(use_graph=self.use_graph)
+        if self.autotune and not getattr(plan, "tuned", False):
+            plan.autotune()
+            plan.tuned = True
         return u_out, p_out
 
     # ------------------------------------------------------- operator path

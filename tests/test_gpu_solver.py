@@ -170,3 +170,22 @@ def test_gpu_reproduces_golden_solve(dev):
     assert np.array_equal(u.cpu().numpy(), g["u_f32"]) and np.array_equal(p.cpu().numpy(), g["p_f32"])
     err64 = np.abs(u.cpu().numpy().astype(np.float64) - g["u_f64"]).max()
     print(f"golden: GPU == FP32 oracle; max |u - u_fp64| = {err64:.2e}")
+
+
+def test_autotune_keeps_results(dev):
+    cfg = SolverConfig(num_scales=2, max_warps=2, max_outer_iterations=4)
+    shapes = [(16, 18, 20), (12, 12, 24)]
+    pairs = [pair(s, seed=30 + i) for i, s in enumerate(shapes)]
+    plan = Plan(cfg, shapes)
+    outs = []
+    for k, (a, b, _) in enumerate(pairs):
+        u = torch.empty(shapes[k] + (3,), device=dev)
+        plan.bind(k, torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev), u_out=u)
+        outs.append(u)
+    plan.run(use_graph=True)
+    before = [u.clone() for u in outs]
+    tuned = plan.autotune(reps=2)
+    assert len(tuned) == 2 and all(bx >= 3 and by >= 3 and cz >= 1 for bx, by, cz in tuned)
+    plan.run(use_graph=True)
+    torch.cuda.synchronize()
+    assert all(torch.equal(a, b) for a, b in zip(before, outs))
