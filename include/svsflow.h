@@ -107,6 +107,14 @@ int svs_prolongate3d(const float* u, int C, int NZ, int NY, int NX, float LZ, fl
  * [K,K,K].  Replaces the F.conv3d call of svs/utils/flow_utils.py:94-112. */
 int svs_blur3d(const float* in, int NZ, int NY, int NX, const float* weights, int K, float* out, void* stream);
 
+/* apply_median_filter: for B flows [B][Z,Y,X,C], per component the median
+ * (element of rank ks^3/2) of the ks^3 window, SciPy 'reflect' extension at
+ * the faces, 1 <= ks <= 7.  Replaces the host round trip through
+ * scipy.ndimage.median_filter(size=(ks,ks,ks,1)) of
+ * svs/utils/flow_utils.py:115-132, applied per pair at
+ * svs/modules/flow/processor.py:116-117.  `out` must not alias `in`. */
+int svs_median3d(const float* in, int B, int NZ, int NY, int NX, int C, int ks, float* out, void* stream);
+
 /* ---------------------------------------------------- the whole solve */
 
 /* Constructor arguments of TVL13DOpticalFlow (svs/modules/flow/tvl13d.py:14-67)

@@ -196,6 +196,18 @@ def blur(vol, weights, dtype=np.float32):
     return out
 
 
+def median3d(u, ks, dtype=np.float32):
+    """scipy.ndimage.median_filter(u, size=(ks,ks,ks,1)) (flow_utils.py:115-132) on
+    [Z,Y,X,C] or a batch [B,Z,Y,X,C]."""
+    u = _arr(u, dtype)
+    batched = u.ndim == 5
+    ub = u if batched else u[None]
+    out = np.empty_like(ub)
+    B, NZ, NY, NX, C = ub.shape
+    _fn("median3d", dtype)(_p(ub), B, NZ, NY, NX, C, int(ks), _p(out))
+    return out if batched else out[0]
+
+
 def solve_params(num_scales=4, max_warps=10, max_outer_iterations=50, weight_matching=1.0, weight_tv=0.025,
                  pd_type=2, pd_params=(0.5, 0.5, 0.25, 1.0), use_aniso=True, aniso_params=(5.0, 0.5),
                  interpolation="CUBIC_HERMITESPLINE", boundary="MIRROR", length_type="numDofs",

@@ -23,6 +23,7 @@
 //   prox primal / dual ....... hand-computed branch cases, idempotence
 //   warp adjoint ............. dot-product identity (image), FP64 central FD (flow)
 //   blur ..................... torch.nn.functional.conv3d on CPU
+//   median filter ............ scipy.ndimage.median_filter on CPU
 //   solve .................... FP64 twin, EPE against a known displacement
 #include <cmath>
 #include <cstdint>
@@ -638,6 +639,46 @@ template <class T> static void solve_level(const SolveParams& P, const Mesh<T>& 
   }
 }
 
+// ------------------------------------------------------- median filter
+// flow_utils.py:115-132: scipy.ndimage.median_filter(u, size=(ks,ks,ks,1)) with
+// its default mode 'reflect' on B flows [B][Z,Y,X,C].  Per component, the
+// window holds the ks^3 values at offsets -(ks/2) .. ks-1-(ks/2) per axis;
+// the result is its element of rank ks^3/2 in ascending order.  Indices
+// outside [0, N) reflect about the face (-1 -> 0, N -> N-1), with period 2N.
+static int reflect_half_sample(int i, int N) {
+  const int P = 2 * N;
+  i %= P;
+  if (i < 0) i += P;
+  return i < N ? i : P - 1 - i;
+}
+
+template <class T>
+static void median3d(const T* in, int B, int NZ, int NY, int NX, int C, int ks, T* out) {
+  const int h = ks / 2, n = ks * ks * ks;
+  const size_t N = (size_t)NZ * NY * NX;
+#pragma omp parallel for schedule(static) collapse(2)
+  for (int b = 0; b < B; ++b)
+    for (int z = 0; z < NZ; ++z) {
+      std::vector<T> win((size_t)n);
+      const T* src = in + (size_t)b * N * C;
+      for (int y = 0; y < NY; ++y)
+        for (int x = 0; x < NX; ++x)
+          for (int c = 0; c < C; ++c) {
+            int m = 0;
+            for (int a = 0; a < ks; ++a)
+              for (int e = 0; e < ks; ++e)
+                for (int f = 0; f < ks; ++f) {
+                  const size_t zz = (size_t)reflect_half_sample(z + a - h, NZ);
+                  const size_t yy = (size_t)reflect_half_sample(y + e - h, NY);
+                  const size_t xx = (size_t)reflect_half_sample(x + f - h, NX);
+                  win[(size_t)m++] = src[((zz * NY + yy) * NX + xx) * C + c];
+                }
+            std::nth_element(win.begin(), win.begin() + n / 2, win.end());
+            out[(size_t)b * N * C + (((size_t)z * NY + y) * NX + x) * C + c] = win[(size_t)(n / 2)];
+          }
+    }
+}
+
 // TVL13DOpticalFlow.compute, tvl13d.py:69-162.  I0/I1 are already blurred
 // when the caller wants the blur (the blur weights come from flow_utils.py).
 template <class T> static void solve(const SolveParams& P, const T* I0, const T* I1, int NZ, int NY, int NX,
@@ -730,6 +771,9 @@ using namespace orc;
   }                                                                                                            \
   extern "C" void orc_blur_##SFX(const T* in, int NZ, int NY, int NX, const T* w, int K, T* out) {              \
     blur<T>(in, NZ, NY, NX, w, K, out);                                                                        \
+  }                                                                                                            \
+  extern "C" void orc_median3d_##SFX(const T* in, int B, int NZ, int NY, int NX, int C, int ks, T* out) {      \
+    median3d<T>(in, B, NZ, NY, NX, C, ks, out);                                                                \
   }                                                                                                            \
   extern "C" void orc_solve_##SFX(const double* params, const T* I0, const T* I1, int NZ, int NY, int NX,       \
                                   const T* u_in, const T* p_in, T* u_out, T* p_out) {                          \

@@ -359,6 +359,123 @@ __global__ __launch_bounds__(kBlock) void k_blur(const float* __restrict__ in, i
   out[i] = acc;
 }
 
+// --------------------------------------------------------- median filter
+// apply_median_filter (flow_utils.py:115-132) is scipy.ndimage.median_filter
+// with size (ks, ks, ks, 1) on a [Z,Y,X,C] flow: for every voxel and
+// component, the element of rank n/2 (0-based, ascending, n = ks^3) of the
+// window at offsets -(ks/2) .. ks-1-(ks/2) per axis.  Outside the volume the
+// window reads SciPy's default 'reflect' extension (d c b a | a b c d | d c b a),
+// which repeats with period 2N.
+constexpr int kMedianMaxK = 7;
+
+__device__ __forceinline__ int reflect_index(int i, int N) {
+  const int P = 2 * N;
+  i %= P;
+  if (i < 0) i += P;
+  return i < N ? i : P - 1 - i;
+}
+
+// Batcher's odd-even merge sort for n inputs (any n), as a comparator list;
+// every comparator moves the smaller value to the lower index.
+template <int N>
+struct OddEvenMergeNet {
+  unsigned char a[N * N], b[N * N];
+  int n;
+  constexpr OddEvenMergeNet() : a{}, b{}, n(0) {
+    for (int p = 1; p < N; p <<= 1)
+      for (int k = p; k >= 1; k >>= 1)
+        for (int j = k % p; j + k < N; j += 2 * k)
+          for (int i = 0; i < k && i + j + k < N; ++i)
+            if ((i + j) / (2 * p) == (i + j + k) / (2 * p)) {
+              a[n] = (unsigned char)(i + j);
+              b[n] = (unsigned char)(i + j + k);
+              ++n;
+            }
+  }
+};
+
+// 3x3x3: the 27 window values in registers through the merge network; only
+// the comparators that feed rank 13 survive dead-code elimination.
+__global__ __launch_bounds__(kBlock) void k_median3(const float* __restrict__ in, int NZ, int NY, int NX, int C,
+                                                    int64_t total, float* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= total) return;
+  const int c = (int)(t % C);
+  const int64_t i = t / C;
+  const int64_t N = (int64_t)NZ * NY * NX;
+  const int64_t b = i / N;
+  int z, y, x;
+  unflatten(i - b * N, NY, NX, z, y, x);
+  const float* src = in + b * N * C + c;
+  int zs[3], ys[3], xs[3];
+#pragma unroll
+  for (int o = 0; o < 3; ++o) {
+    zs[o] = reflect_index(z + o - 1, NZ);
+    ys[o] = reflect_index(y + o - 1, NY);
+    xs[o] = reflect_index(x + o - 1, NX);
+  }
+  float v[27];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+        v[(a * 3 + e) * 3 + f] = src[(((int64_t)zs[a] * NY + ys[e]) * NX + xs[f]) * C];
+  constexpr OddEvenMergeNet<27> net;
+#pragma unroll
+  for (int q = 0; q < net.n; ++q) {
+    const float lo = fminf(v[net.a[q]], v[net.b[q]]);
+    const float hi = fmaxf(v[net.a[q]], v[net.b[q]]);
+    v[net.a[q]] = lo;
+    v[net.b[q]] = hi;
+  }
+  out[t] = v[13];
+}
+
+// Any ks up to kMedianMaxK: the window in a private array, the answer is the
+// value whose count of smaller elements is <= n/2 < its count of smaller or
+// equal elements.
+__global__ __launch_bounds__(kBlock) void k_median_any(const float* __restrict__ in, int NZ, int NY, int NX, int C,
+                                                       int ks, int64_t total, float* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= total) return;
+  const int c = (int)(t % C);
+  const int64_t i = t / C;
+  const int64_t N = (int64_t)NZ * NY * NX;
+  const int64_t b = i / N;
+  int z, y, x;
+  unflatten(i - b * N, NY, NX, z, y, x);
+  const float* src = in + b * N * C + c;
+  float win[kMedianMaxK * kMedianMaxK * kMedianMaxK];
+  const int h = ks / 2, n = ks * ks * ks;
+  int m = 0;
+  for (int a = 0; a < ks; ++a) {
+    const int zz = reflect_index(z + a - h, NZ);
+    for (int e = 0; e < ks; ++e) {
+      const int yy = reflect_index(y + e - h, NY);
+      for (int f = 0; f < ks; ++f) {
+        const int xx = reflect_index(x + f - h, NX);
+        win[m++] = src[(((int64_t)zz * NY + yy) * NX + xx) * C];
+      }
+    }
+  }
+  const int rank = n / 2;
+  float res = win[0];
+  for (int q = 0; q < n; ++q) {
+    int less = 0, leq = 0;
+    for (int r = 0; r < n; ++r) {
+      less += win[r] < win[q];
+      leq += win[r] <= win[q];
+    }
+    if (less <= rank && rank < leq) {
+      res = win[q];
+      break;
+    }
+  }
+  out[t] = res;
+}
+
 int launch_fill(float* p, int64_t n, float v, hipStream_t s) {
   if (n <= 0) return SVS_STATUS_OK;
   k_fill<<<grid_for(n, kBlock), kBlock, 0, s>>>(p, n, v);
@@ -510,6 +627,23 @@ int svs_blur3d(const float* in, int NZ, int NY, int NX, const float* weights, in
   SVS_REQUIRE_DIMS(NZ, NY, NX);
   const int64_t n = (int64_t)NZ * NY * NX;
   k_blur<<<grid_for(n, kBlock), kBlock, 0, (hipStream_t)stream>>>(in, NZ, NY, NX, weights, K, out);
+  SVS_LAUNCHED();
+  return SVS_STATUS_OK;
+}
+
+int svs_median3d(const float* in, int B, int NZ, int NY, int NX, int C, int ks, float* out, void* stream) {
+  SVS_REQUIRE(in && out, "apply_median_filter: null pointer");
+  SVS_REQUIRE(in != out, "apply_median_filter: the output must not alias the input");
+  SVS_REQUIRE(B >= 1 && C >= 1, "apply_median_filter: batch and component counts must be positive (got %d, %d)", B, C);
+  SVS_REQUIRE(ks >= 1 && ks <= kMedianMaxK, "apply_median_filter: kernel size must be in [1, %d] (got %d)",
+              kMedianMaxK, ks);
+  SVS_REQUIRE_DIMS(NZ, NY, NX);
+  const int64_t total = (int64_t)B * NZ * NY * NX * C;
+  SVS_REQUIRE(total < ((int64_t)1 << 38), "apply_median_filter: too many elements");
+  if (ks == 3)
+    k_median3<<<grid_for(total, kBlock), kBlock, 0, (hipStream_t)stream>>>(in, NZ, NY, NX, C, total, out);
+  else
+    k_median_any<<<grid_for(total, kBlock), kBlock, 0, (hipStream_t)stream>>>(in, NZ, NY, NX, C, ks, total, out);
   SVS_LAUNCHED();
   return SVS_STATUS_OK;
 }

@@ -261,7 +261,10 @@ struct alignas(16) SweepTile {
   int inst, x0, y0, zr;  // zr = z0 | (z1 << 16)
 };
 
-template <int BC, bool ANISO>
+// UNIT_H: every instance of the launch has h = 1 on all three axes (always
+// so for numDofs meshes, h = (N-1)/(N-1)).  The divisions by h then drop out
+// at compile time; x / 1.0f == x, so the results are the same bits.
+template <int BC, bool ANISO, bool UNIT_H>
 __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ descs,
                                                 const SweepTile* __restrict__ tiles, int parity, float sigma,
                                                 float tau, float factor, float wtv, int wtv_zero) {
@@ -270,7 +273,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   const LevelDesc& d = descs[t.inst];
   const int NZ = d.NZ, NY = d.NY, NX = d.NX;
   const int64_t N = d.N;
-  const float hZ = d.hZ, hY = d.hY, hX = d.hX;
+  const float hZ = UNIT_H ? 1.0f : d.hZ, hY = UNIT_H ? 1.0f : d.hY, hX = UNIT_H ? 1.0f : d.hX;
   const int BX = blockDim.x, BY = blockDim.y;
   const int ex = threadIdx.x, ey = threadIdx.y;
   const int x = t.x0 - 1 + ex, y = t.y0 - 1 + ey;
@@ -285,23 +288,22 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   const int ro = ey * BX + ex;
   const int tid = ey * BX + ex;
 
-  // Global addressing: a wave-uniform plane base plus this thread's 32-bit
-  // in-plane offset (the host keeps NY*NX < 2^31).
-  const int64_t plane = (int64_t)NY * NX;
+  // Buffer addressing: this thread's in-plane byte offset in a VGPR, the
+  // field and plane offsets in SGPRs (the host keeps 36*N < 2^32).
+  const uint32_t F4 = (uint32_t)N * 4u;              // bytes of one scalar field
+  const uint32_t P4 = (uint32_t)(NY * NX) * 4u;      // bytes of one z-plane of it
   const uint32_t col = valid ? (uint32_t)(y * NX + x) : 0u;
-  const SVS_GLOBAL float* __restrict__ uin = as_global((const float*)d.u[parity]);
-  SVS_GLOBAL float* __restrict__ uout = as_global(d.u[parity ^ 1]);
-  const SVS_GLOBAL float* __restrict__ pin = as_global((const float*)d.p[parity]);
-  SVS_GLOBAL float* __restrict__ pout = as_global(d.p[parity ^ 1]);
-  const SVS_GLOBAL float4* __restrict__ SNg = as_global((const float4*)d.SN);
-  const SVS_GLOBAL float4* __restrict__ RGg = as_global((const float4*)d.RG);
+  const uint32_t vo = col * 4u, vo4 = col * 16u;
+  const svs_rsrc uin = make_rsrc(d.u[parity], 3u * F4), uout = make_rsrc(d.u[parity ^ 1], 3u * F4);
+  const svs_rsrc pin = make_rsrc(d.p[parity], 9u * F4), pout = make_rsrc(d.p[parity ^ 1], 9u * F4);
+  const svs_rsrc SNr = make_rsrc(d.SN, 4u * F4), RGr = make_rsrc(d.RG, 4u * F4);
 
   // This thread's position on the outer ring of the u tile (one per thread;
   // the host guarantees 2*UW + 2*BY <= BX*BY).
   const int nring = 2 * UW + 2 * BY;
   const bool ring = tid < nring;
   int ring_lds = 0;
-  uint32_t ring_col = 0;
+  uint32_t ring_vo = 0;
   bool ring_ok = false;
   if (ring) {
     int ux, uy;
@@ -315,7 +317,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
     }
     const int gx = t.x0 - 2 + ux, gy = t.y0 - 2 + uy;
     ring_ok = gx >= 0 && gx < NX && gy >= 0 && gy < NY;
-    ring_col = ring_ok ? (uint32_t)(gy * NX + gx) : 0u;
+    ring_vo = ring_ok ? (uint32_t)(gy * NX + gx) * 4u : 0u;
     ring_lds = uy * UW + ux;
   }
 
@@ -330,57 +332,44 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   float pc[9], snc[4] = {0.0f, 0.0f, 0.0f, 0.0f}, rgc[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ringc[3] = {0.0f, 0.0f, 0.0f};
   for (int e = 0; e < 9; ++e) pc[e] = 0.0f;
   if (valid) {
-    if (kstart >= 1) {
-      const SVS_GLOBAL float* b = uin + (kstart - 1) * plane;
-      for (int c = 0; c < 3; ++c) um[c] = (b + c * N)[col];
-    }
-    const SVS_GLOBAL float* b0 = uin + kstart * plane;
-    for (int c = 0; c < 3; ++c) uc[c] = (b0 + c * N)[col];
-    if (kstart + 1 < NZ) {
-      const SVS_GLOBAL float* b1 = uin + (kstart + 1) * plane;
-      for (int c = 0; c < 3; ++c) up[c] = (b1 + c * N)[col];
-    }
-    const SVS_GLOBAL float* pb = pin + kstart * plane;
-    for (int e = 0; e < 9; ++e) pc[e] = (pb + e * N)[col];
+    if (kstart >= 1)
+      for (int c = 0; c < 3; ++c) um[c] = bld(uin, vo, c * F4 + (kstart - 1) * P4);
+    for (int c = 0; c < 3; ++c) uc[c] = bld(uin, vo, c * F4 + kstart * P4);
+    if (kstart + 1 < NZ)
+      for (int c = 0; c < 3; ++c) up[c] = bld(uin, vo, c * F4 + (kstart + 1) * P4);
+    for (int e = 0; e < 9; ++e) pc[e] = bld(pin, vo, e * F4 + kstart * P4);
     if (ANISO) {
-      const svs_f4 sn = ld4(SNg + kstart * plane, col);
+      const svs_f4 sn = bld4(SNr, vo4, 4u * kstart * P4);
       snc[0] = sn.x; snc[1] = sn.y; snc[2] = sn.z; snc[3] = sn.w;
     }
   }
-  if (ring_ok) {
-    const SVS_GLOBAL float* b = uin + kstart * plane;
-    for (int c = 0; c < 3; ++c) ringc[c] = (b + c * N)[ring_col];
-  }
+  if (ring_ok)
+    for (int c = 0; c < 3; ++c) ringc[c] = bld(uin, ring_vo, c * F4 + kstart * P4);
   float r2a[3] = {0.0f, 0.0f, 0.0f}, r2b[3] = {0.0f, 0.0f, 0.0f}, r2c[3];
 
   for (int k = kstart; k <= kend; ++k) {
     const int buf = k & 1;
-    // ---- prefetch what step k+1 needs; consumed after this step's barrier
-    float un[3] = {0.0f, 0.0f, 0.0f}, pn[9], snn[4] = {0.0f, 0.0f, 0.0f, 0.0f}, rgn[4] = {0.0f, 0.0f, 0.0f, 0.0f},
-          ringn[3] = {0.0f, 0.0f, 0.0f};
-    for (int e = 0; e < 9; ++e) pn[e] = 0.0f;
-    if (valid) {
-      if (k + 2 < NZ) {
-        const SVS_GLOBAL float* b = uin + (k + 2) * plane;
-        for (int c = 0; c < 3; ++c) un[c] = (b + c * N)[col];
-      }
-      if (k + 1 < NZ) {
-        const SVS_GLOBAL float* pb = pin + (k + 1) * plane;
-        for (int e = 0; e < 9; ++e) pn[e] = (pb + e * N)[col];
-        if (ANISO) {
-          const svs_f4 sn = ld4(SNg + (k + 1) * plane, col);
-          snn[0] = sn.x; snn[1] = sn.y; snn[2] = sn.z; snn[3] = sn.w;
-        }
-      }
+    // ---- prefetch what step k+1 needs; consumed after this step's barrier.
+    // Plane indices are clamped into the volume and lanes outside it read
+    // in-plane offset 0, so every load is in bounds and needs no branch: a
+    // clamped or outside value only reaches steps that are skipped (k >= NZ,
+    // outside z0..z1) or derivative branches the boundary rule does not select
+    // (the last plane's upper neighbour, a face's outer neighbour).
+    const uint32_t k1 = (uint32_t)min(k + 1, NZ - 1), k2 = (uint32_t)min(k + 2, NZ - 1);
+    float un[3], pn[9], snn[4] = {0.0f, 0.0f, 0.0f, 0.0f}, rgn[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ringn[3];
+    for (int c = 0; c < 3; ++c) un[c] = bld(uin, vo, c * F4 + k2 * P4);
+    for (int e = 0; e < 9; ++e) pn[e] = bld(pin, vo, e * F4 + k1 * P4);
+    if (ANISO) {
+      const svs_f4 sn = bld4(SNr, vo4, 4u * k1 * P4);
+      snn[0] = sn.x; snn[1] = sn.y; snn[2] = sn.z; snn[3] = sn.w;
     }
-    if (interior && k >= z0 && k < z1) {
-      const svs_f4 rg = ld4(RGg + k * plane, col);
+    if (interior) {
+      const svs_f4 rg = bld4(RGr, vo4, 4u * (uint32_t)min(k, NZ - 1) * P4);
       rgn[0] = rg.x; rgn[1] = rg.y; rgn[2] = rg.z; rgn[3] = rg.w;
     }
-    if (ring_ok && k + 1 < NZ) {
-      const SVS_GLOBAL float* b = uin + (k + 1) * plane;
-      for (int c = 0; c < 3; ++c) ringn[c] = (b + c * N)[ring_col];
-    }
+    for (int c = 0; c < 3; ++c) ringn[c] = 0.0f;
+    if (ring)
+      for (int c = 0; c < 3; ++c) ringn[c] = bld(uin, ring_vo, c * F4 + k1 * P4);
 
     // ---- u of plane k into LDS
     if (k < NZ) {
@@ -401,7 +390,6 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
         tangent1(n, t1);
       }
       const bool store_p = interior && k >= z0 && k < z1;
-      SVS_GLOBAL float* pob = pout + k * plane;
       for (int c = 0; c < 3; ++c) {
         const float* Uc = Ub + c * UPL;
         const float b0 = Uc[uo];
@@ -420,7 +408,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
         prox_dual_row(pv, wtv);
         if (wtv_zero) { pv[0] = 0.0f; pv[1] = 0.0f; pv[2] = 0.0f; }
         if (store_p)
-          for (int e = 0; e < 3; ++e) (pob + (3 * c + e) * N)[col] = pv[e];
+          for (int e = 0; e < 3; ++e) bst(pout, vo, (3 * c + e) * F4 + k * P4, pv[e]);
         float rr[3];
         if (ANISO) aniso_row(s, n, t1, pv, rr);
         else { rr[0] = pv[0]; rr[1] = pv[1]; rr[2] = pv[2]; }
@@ -455,8 +443,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
       const float sm = (um[0] * g[0] + um[1] * g[1]) + um[2] * g[2];
       const float rho = rgc[0] + sm;
       prox_primal_point(v, rho, g, factor);
-      SVS_GLOBAL float* uob = uout + km * plane;
-      for (int c = 0; c < 3; ++c) (uob + c * N)[col] = v[c];
+      for (int c = 0; c < 3; ++c) bst(uout, vo, c * F4 + km * P4, v[c]);
     }
 
     for (int c = 0; c < 3; ++c) {
@@ -482,6 +469,7 @@ struct LevelPlan {
   int ntiles = 0;
   int tile_capacity = 0;
   int bx = 0, by = 0, cz = 0;
+  bool unit_h = false;           // h == 1 on every axis of every instance
 };
 
 }  // namespace svs
@@ -703,8 +691,13 @@ static int plan_layout(svs_plan* pl, char* base, size_t* total_out) {
     L.tile_capacity = cap;
     L.tiles = (SweepTile*)take(sizeof(SweepTile) * cap);
     int nb = 0;
-    for (auto& d : L.host) nb += grid_for(d.N, kBlock);
+    bool unit = true;
+    for (auto& d : L.host) {
+      nb += grid_for(d.N, kBlock);
+      unit = unit && d.hZ == 1.0f && d.hY == 1.0f && d.hX == 1.0f;
+    }
     L.nblocks = nb;
+    L.unit_h = unit;
   }
   if (P.use_blur) pl->blur_dev = (float*)take(sizeof(float) * pl->blur_w.size());
   *total_out = align_up(off, 256);
@@ -727,6 +720,11 @@ static int validate_dims(const svs_tvl1_params* P, int ninst, const int* dims) {
     int nz = dims[3 * k], ny = dims[3 * k + 1], nx = dims[3 * k + 2];
     SVS_REQUIRE_DIMS(nz, ny, nx);
     SVS_REQUIRE(nz < 32768, "NZ must be < 32768");
+    // The fused sweep addresses the 9 dual fields of an instance with 32-bit
+    // byte offsets.
+    SVS_REQUIRE((int64_t)nz * ny * nx <= (int64_t)0xFFFFFFFF / 36,
+                "instance %d: %dx%dx%d exceeds the %lld voxels a plan instance may hold", k, nz, ny, nx,
+                (long long)(0xFFFFFFFF / 36));
     for (int s = 0; s < P->num_scales; ++s) {
       if (s > 0) { nz = (nz + 1) / 2; ny = (ny + 1) / 2; nx = (nx + 1) / 2; }
       SVS_REQUIRE(nz >= 4 && ny >= 4 && nx >= 4,
@@ -838,8 +836,12 @@ template <int BC, bool ANISO>
 static void launch_sweep_t(const LevelPlan& L, int parity, float sg, float ta, float fac, float wtv, int wz,
                            hipStream_t st) {
   const size_t lds = sweep_lds_bytes(L.bx, L.by);
-  hipLaunchKernelGGL((k_sweep<BC, ANISO>), dim3(L.ntiles), dim3(L.bx, L.by), lds, st, L.dev, L.tiles, parity, sg,
-                     ta, fac, wtv, wz);
+  if (L.unit_h)
+    hipLaunchKernelGGL((k_sweep<BC, ANISO, true>), dim3(L.ntiles), dim3(L.bx, L.by), lds, st, L.dev, L.tiles,
+                       parity, sg, ta, fac, wtv, wz);
+  else
+    hipLaunchKernelGGL((k_sweep<BC, ANISO, false>), dim3(L.ntiles), dim3(L.bx, L.by), lds, st, L.dev, L.tiles,
+                       parity, sg, ta, fac, wtv, wz);
 }
 
 static void launch_sweep(const svs_plan* pl, const LevelPlan& L, int parity, int n, hipStream_t st) {
@@ -864,12 +866,14 @@ static int allow_large_lds(const svs_plan* pl) {
   size_t need = 96 * 1024;
   for (const LevelPlan& L : pl->levels) need = std::max(need, sweep_lds_bytes(L.bx, L.by));
   const int v = (int)need;
-  SVS_HIP(hipFuncSetAttribute((const void*)k_sweep<BD_NEAREST, false>, hipFuncAttributeMaxDynamicSharedMemorySize, v));
-  SVS_HIP(hipFuncSetAttribute((const void*)k_sweep<BD_NEAREST, true>, hipFuncAttributeMaxDynamicSharedMemorySize, v));
-  SVS_HIP(hipFuncSetAttribute((const void*)k_sweep<BD_MIRROR, false>, hipFuncAttributeMaxDynamicSharedMemorySize, v));
-  SVS_HIP(hipFuncSetAttribute((const void*)k_sweep<BD_MIRROR, true>, hipFuncAttributeMaxDynamicSharedMemorySize, v));
-  SVS_HIP(hipFuncSetAttribute((const void*)k_sweep<BD_REFLECT, false>, hipFuncAttributeMaxDynamicSharedMemorySize, v));
-  SVS_HIP(hipFuncSetAttribute((const void*)k_sweep<BD_REFLECT, true>, hipFuncAttributeMaxDynamicSharedMemorySize, v));
+  const void* kernels[] = {
+      (const void*)k_sweep<BD_NEAREST, false, false>, (const void*)k_sweep<BD_NEAREST, true, false>,
+      (const void*)k_sweep<BD_MIRROR, false, false>,  (const void*)k_sweep<BD_MIRROR, true, false>,
+      (const void*)k_sweep<BD_REFLECT, false, false>, (const void*)k_sweep<BD_REFLECT, true, false>,
+      (const void*)k_sweep<BD_NEAREST, false, true>,  (const void*)k_sweep<BD_NEAREST, true, true>,
+      (const void*)k_sweep<BD_MIRROR, false, true>,   (const void*)k_sweep<BD_MIRROR, true, true>,
+      (const void*)k_sweep<BD_REFLECT, false, true>,  (const void*)k_sweep<BD_REFLECT, true, true>};
+  for (const void* k : kernels) SVS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, v));
   return 0;
 }
 

@@ -223,8 +223,12 @@ __device__ __forceinline__ void prox_dual_row(float v[3], float wtv) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) ns += v[d] * v[d];
   const float den = fmaxf(1.0f, sqrtf(ns) / wtv);
+  // Inside the ball den == 1 and v / 1 == v exactly: a wave whose rows all lie
+  // inside skips the three divisions.
+  if (den > 1.0f) {
 #pragma unroll
-  for (int d = 0; d < 3; ++d) v[d] = v[d] / den;
+    for (int d = 0; d < 3; ++d) v[d] = v[d] / den;
+  }
 }
 
 // opticalFlowOps.cu:131-173 — the L1 data-term shift added to the primal
@@ -264,6 +268,26 @@ __device__ __forceinline__ SVS_GLOBAL T* as_global(T* p) {
 typedef float svs_f4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ svs_f4 ld4(const SVS_GLOBAL float4* p, uint32_t i) {
   return ((const SVS_GLOBAL svs_f4*)p)[i];
+}
+
+// Raw buffer access for the fused sweep.  A 128-bit resource (base address and
+// size in bytes) sits in scalar registers; each access adds a per-lane byte
+// offset (VGPR) and a wave-uniform byte offset (SGPR), so the plane and field
+// offsets cost no vector instructions.  Reads at or past `bytes` return 0 and
+// writes there are dropped.  0x00020000 is the gfx9 word 3 (32-bit data format).
+typedef __amdgpu_buffer_rsrc_t svs_rsrc;
+__device__ __forceinline__ svs_rsrc make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float bld(svs_rsrc r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+typedef unsigned int svs_u4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ svs_f4 bld4(svs_rsrc r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(svs_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void bst(svs_rsrc r, uint32_t voff, uint32_t soff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, (int)soff, 0);
 }
 
 // Linear voxel index -> (z, y, x).

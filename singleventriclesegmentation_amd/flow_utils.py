@@ -3,8 +3,9 @@
 The Gaussian pre-blur runs on the GPU (svs_blur3d); its weights are built on
 the host with the same torch calls the reference makes, so the kernel taps
 are the reference's (including the 1.5-voxel tap spacing of the linspace at
-flow_utils.py:87 for sigma = 1).  The median post-filter stays a SciPy host
-round trip exactly as in the reference (flow_utils.py:115-132).
+flow_utils.py:87 for sigma = 1).  The median post-filter (flow_utils.py:115-132,
+a SciPy host round trip in the reference) runs on the GPU as well
+(svs_median3d), with SciPy's window, rank and 'reflect' extension.
 """
 import enum
 from typing import Literal, Tuple, Union
@@ -97,11 +98,33 @@ def apply_gaussian_blur3d(vol: torch.Tensor, sigma: float) -> torch.Tensor:
     return out
 
 
+def median_filter3d(u: torch.Tensor, ks: int) -> torch.Tensor:
+    """Per-component median over a ks^3 window of a [Z,Y,X,C] flow or a batch
+    [B,Z,Y,X,C] of them, on the tensor's GPU; a new float32 tensor.
+
+    The result is scipy.ndimage.median_filter(u, size=(ks,ks,ks,1)) (per flow):
+    the element of rank ks^3 // 2 of each window, with SciPy's default
+    'reflect' extension at the faces."""
+    if u.device.type != "cuda" or u.dtype != torch.float32 or u.dim() not in (4, 5):
+        raise RuntimeError("median filter expects a float32 [Z,Y,X,C] or [B,Z,Y,X,C] CUDA tensor")
+    ub = (u if u.dim() == 5 else u.unsqueeze(0)).contiguous()
+    B, NZ, NY, NX, C = ub.shape
+    with torch.cuda.device(ub.device):
+        out = torch.empty_like(ub)
+        check(_lib.lib().svs_median3d(ptr(ub), B, NZ, NY, NX, C, int(ks), ptr(out), stream_ptr()))
+    return out if u.dim() == 5 else out[0]
+
+
 def apply_median_filter(u: torch.Tensor, ks: int, device: torch.device) -> torch.Tensor:
-    """SciPy median over (ks, ks, ks, 1), mode 'reflect' (flow_utils.py:115-132), via the host."""
-    from scipy import ndimage
-    uf = ndimage.median_filter(u.cpu().detach().numpy(), size=(ks, ks, ks, 1))
-    return torch.from_numpy(uf).float().to(device)
+    """flow_utils.py:115-132: median over (ks, ks, ks, 1) of a [Z,Y,X,C] flow,
+    returned on ``device``.  Computed on the GPU (``device`` must be a CUDA
+    device) instead of through SciPy on the host; the values are SciPy's."""
+    device = torch.device(device)
+    if device.type != "cuda":
+        raise RuntimeError("apply_median_filter runs on a CUDA device (got %s)" % device)
+    if u.dim() != 4:
+        raise RuntimeError("apply_median_filter expects a [Z,Y,X,C] tensor (size (ks, ks, ks, 1))")
+    return median_filter3d(u.detach().to(device=device, dtype=torch.float32), ks)
 
 
 def compute_timepoints(dia_ts: int, sys_ts: int, seg_dia, seg_sys, mode: Union[str, FlowDirection]):

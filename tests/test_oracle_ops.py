@@ -299,3 +299,19 @@ def test_shared_expf_powf_are_faithful(O):
     ys = np.concatenate([R.uniform(-3, 3, 3000), [0.5, 0.5, 0.5]]).astype(np.float32)
     for x, y in zip(xs, ys):
         assert ulps(L.orc_svs_powf(float(x), float(y)), np.float32(float(x) ** float(y))) <= 1
+
+
+@pytest.mark.parametrize("shape,ks", [((5, 6, 7, 3), 3), ((4, 4, 4, 3), 5), ((6, 5, 4, 2), 4), ((3, 3, 3, 1), 7),
+                                      ((9, 8, 7, 3), 1), ((2, 3, 4, 3), 3), ((1, 5, 6, 3), 2)])
+def test_median_matches_scipy(O, shape, ks):
+    """flow_utils.py:131 calls scipy.ndimage.median_filter(u, size=(ks,ks,ks,1)) with
+    its default 'reflect' mode; ties (rounded values) and windows wider than the
+    volume (period-2N reflection) included."""
+    from scipy import ndimage
+    u = R.standard_normal(shape).astype(np.float32)
+    u[..., 0] = np.round(u[..., 0])
+    ref = ndimage.median_filter(u, size=(ks, ks, ks, 1))
+    assert np.array_equal(O.median3d(u, ks), ref)
+    batch = np.stack([u, -u])
+    got = O.median3d(batch, ks)
+    assert np.array_equal(got[0], ref) and np.array_equal(got[1], ndimage.median_filter(-u, size=(ks, ks, ks, 1)))
