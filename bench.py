@@ -33,8 +33,13 @@ def parse():
     ap.add_argument("--frames", type=int, default=35)
     ap.add_argument("--size", type=int, default=80)
     ap.add_argument("--plain-pairs", type=int, default=6, help="pairs timed through the plain drop-in")
+    ap.add_argument("--operator-pairs", type=int, default=1,
+                    help="pairs timed through the reference's step-by-step driver over the drop-in operators")
     ap.add_argument("--sweep-launches", type=int, default=100, help="sweeps per level timed for the roofline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep-shapes", default="",
+                    help="fixed fused-sweep shapes 'bx,by,cz;...' per level instead of autotuning "
+                         "(reproduces a measured configuration, e.g. under a profiler)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference kernel for PMC byte counters)")
     ap.add_argument("--workload", default="sequence", choices=["sequence", "uneven", "serve", "unet"],
@@ -45,6 +50,28 @@ def parse():
     ap.add_argument("--rates", default="5,20,50", help="serve: arrival rates (req/s)")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
+
+
+PMC_SUMMARY = os.path.join("profiles", "k_sweep_68x80cubed_pmc.md")
+
+
+def pmc_traffic(size, npairs):
+    """HBM bytes per fused-sweep launch (mean over the levels) measured with
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload, scaled against a
+    device copy; read from the committed summary (counters cannot be read from
+    inside the timed process).  None for any other workload."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), PMC_SUMMARY)
+    if size != 80 or npairs != 68 or not os.path.exists(path):
+        return None, None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            key, _, val = line.partition(":")
+            if key in ("traffic_bytes_per_launch_avg", "traffic_shapes"):
+                vals[key] = val.strip()
+    if "traffic_bytes_per_launch_avg" not in vals:
+        return None, None
+    return int(vals["traffic_bytes_per_launch_avg"]), vals.get("traffic_shapes")
 
 
 def log(args, *a):
@@ -102,10 +129,15 @@ def main():
         bwd = uf[T - 1:T - 1 + 12].flip(0)  # flows 12->11 ... 1->0
         return propagate(warper, m0, m12, fwd, bwd)
 
-    log(args, "warm-up and sweep-shape autotuning")
     step()
-    tuned = plan.autotune()
-    log(args, "tuned sweep shapes (bx, by, cz) per level:", tuned)
+    if args.sweep_shapes:
+        tuned = [tuple(int(v) for v in t.split(",")) for t in args.sweep_shapes.split(";")]
+        plan.set_shapes(tuned)
+        log(args, "fixed sweep shapes (bx, by, cz) per level:", tuned)
+    else:
+        log(args, "warm-up and sweep-shape autotuning")
+        tuned = plan.autotune()
+        log(args, "tuned sweep shapes (bx, by, cz) per level:", tuned)
     for _ in range(args.warmup):
         step()  # the first step after tuning re-captures the graph
     torch.cuda.synchronize()
@@ -138,6 +170,7 @@ def main():
     avg_bytes = sum(sweep_bytes) / len(levels)
     avg_ms = sum(sweep_ms) / len(levels)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic, traffic_shapes = pmc_traffic(args.size, npairs)
     level0_gbps = sweep_bytes[0] / (sweep_ms[0] * 1e-3) / 1e9
     log(args, "sweep per level (us/launch):", [round(1e3 * v, 1) for v in sweep_ms])
 
@@ -158,6 +191,21 @@ def main():
         # the batched instance computes the same pair: identical bits
         same = bool(torch.equal(u, filtered["u"][args.plain_pairs - 1]))
         log(args, f"plain drop-in: {plain_sps:.2f} solves/s; equal to batched result: {same}")
+
+    # The reference's own driver structure (tvl13d.py:122-255: one operator
+    # call per reference call, ~15 launches per iteration) over the drop-in
+    # operators: the per-pair path a user gets by swapping the import only.
+    op_sps = None
+    if args.operator_pairs > 0:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for j in range(args.operator_pairs):
+            uo, _ = plain.compute_unfused(seq[j + 1], seq[j], None, None)
+        torch.cuda.synchronize()
+        op_sps = args.operator_pairs / (time.perf_counter() - t0)
+        diff = (uo - u_all[args.operator_pairs - 1]).abs().max().item()
+        log(args, f"operator path: {op_sps:.3f} solves/s; max |u - u_batched| = {diff:.2e} (torch's own "
+                  f"elementwise/sum kernels in the reference driver round in their own order)")
 
     if args.copy_mib > 0:
         src = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=dev).uniform_()
@@ -203,10 +251,14 @@ def main():
                    "sweep_shapes_bx_by_cz": [list(t) for t in tuned]},
         "plain_solves_per_s": round(plain_sps, 3) if plain_sps else None,
         "batched_over_plain": round(solves_per_s / plain_sps, 3) if plain_sps else None,
+        "operator_path_solves_per_s": round(op_sps, 4) if op_sps else None,
+        "batched_over_operator_path": round(solves_per_s / op_sps, 1) if op_sps else None,
         "voxel_iterations_per_s": round(solves_per_s * vox_it, 1),
         "algorithmic_GBps": round(solves_per_s * algorithmic_bytes(cfg, shape) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "traffic_source": (f"{PMC_SUMMARY}: 2 x FETCH_SIZE + WRITE_SIZE per launch, mean of the "
+                                        f"levels, sweep shapes {traffic_shapes}") if traffic else None,
                      "kernel": "k_sweep (fused primal-dual sweep)",
                      "bytes_per_launch_avg": int(avg_bytes), "us_per_launch_avg": round(avg_ms * 1e3, 2),
                      "level0_GBps": round(level0_gbps, 1),

@@ -187,6 +187,13 @@ int svs_tvl1_plan_sweeps(svs_plan* plan, int level, int count, void* stream);
  * counterpart. */
 int svs_tvl1_plan_autotune(svs_plan* plan, int reps, void* stream, int* report);
 
+/* Sets the fused-sweep shape of one level (work-group bx x by threads, a
+ * (bx-2) x (by-2) tile; z-chunk cz) and drops the captured graph: for
+ * reproducing a measured configuration.  Fails if the shape exceeds the
+ * kernel's limits or the tile table the plan reserved.  Results do not
+ * depend on the shape.  No reference counterpart. */
+int svs_tvl1_plan_set_shape(svs_plan* plan, int level, int bx, int by, int cz, void* stream);
+
 /* One frame pair without a graph: TVL13DOpticalFlow.compute(I0, I1, u, p). */
 int svs_tvl1_solve(const svs_tvl1_params* params, const float* I0, const float* I1, int NZ, int NY, int NX,
                    const float* u_in, const float* p_in, float* u_out, float* p_out, void* workspace,

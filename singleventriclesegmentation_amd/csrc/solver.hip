@@ -1113,6 +1113,20 @@ int svs_tvl1_plan_autotune(svs_plan* pl, int reps, void* stream, int* report /* 
   return rc;
 }
 
+int svs_tvl1_plan_set_shape(svs_plan* pl, int level, int bx, int by, int cz, void* stream) {
+  SVS_REQUIRE(pl && level >= 0 && level < pl->S, "bad plan/level");
+  SVS_REQUIRE(shape_ok(bx, by, cz), "sweep shape %dx%d, z-chunk %d is outside the kernel's limits", bx, by, cz);
+  LevelPlan& L = pl->levels[level];
+  SVS_REQUIRE(count_tiles(L, bx, by, cz) <= L.tile_capacity,
+              "sweep shape %dx%d, z-chunk %d needs %d tiles; the plan reserved %d", bx, by, cz,
+              count_tiles(L, bx, by, cz), L.tile_capacity);
+  L.bx = bx; L.by = by; L.cz = cz;
+  if (int rc = upload_tiles(pl, level, (hipStream_t)stream)) return rc;
+  if (pl->exec) { (void)hipGraphExecDestroy(pl->exec); pl->exec = nullptr; }
+  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  return 0;
+}
+
 // One frame pair, enqueued directly (no graph): the plain drop-in.
 int svs_tvl1_solve(const svs_tvl1_params* P, const float* I0, const float* I1, int NZ, int NY, int NX,
                    const float* u_in, const float* p_in, float* u_out, float* p_out, void* workspace,

@@ -166,6 +166,15 @@ class Plan:
             check(_lib.lib().svs_tvl1_plan_autotune(self._h, reps, stream_ptr(), rep))
         return [tuple(rep[3 * i:3 * i + 3]) for i in range(self.cfg.num_scales)]
 
+    def set_shapes(self, shapes):
+        """Sets the fused-sweep shape (bx, by, cz) of every level, e.g. to
+        reproduce a configuration the autotuner picked earlier."""
+        if len(shapes) != self.cfg.num_scales:
+            raise RuntimeError(f"need one (bx, by, cz) per level ({self.cfg.num_scales})")
+        with torch.cuda.device(self.device):
+            for lv, (bx, by, cz) in enumerate(shapes):
+                check(_lib.lib().svs_tvl1_plan_set_shape(self._h, lv, int(bx), int(by), int(cz), stream_ptr()))
+
     def sweeps(self, level, count):
         """Enqueues ``count`` fused sweeps of one level (timing aid; needs a prior run)."""
         with torch.cuda.device(self.device):

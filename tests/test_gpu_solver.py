@@ -189,3 +189,28 @@ def test_autotune_keeps_results(dev):
     plan.run(use_graph=True)
     torch.cuda.synchronize()
     assert all(torch.equal(a, b) for a, b in zip(before, outs))
+
+
+def test_set_shapes_keeps_results_and_rejects_bad_shapes(dev):
+    """Fixed per-level sweep shapes (bench --sweep-shapes) re-tile the launch
+    and drop the graph; the results stay bit-identical."""
+    cfg = SolverConfig(num_scales=2, max_warps=2, max_outer_iterations=4)
+    shape = (20, 22, 24)
+    a, b, _ = pair(shape, seed=41)
+    plan = Plan(cfg, [shape])
+    u = torch.empty(shape + (3,), device=dev)
+    plan.bind(0, torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev), u_out=u)
+    plan.run(use_graph=True)
+    before = u.clone()
+    for shapes in ([(26, 6, 8), (14, 6, 3)], [(10, 10, 20), (8, 8, 2)]):
+        plan.set_shapes(shapes)
+        assert plan.info(0)["bx"] == shapes[0][0] and plan.info(1)["cz"] == shapes[1][2]
+        plan.run(use_graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(u, before)
+    with pytest.raises(RuntimeError):
+        plan.set_shapes([(2, 6, 8), (14, 6, 3)])        # bx < 3
+    with pytest.raises(RuntimeError):
+        plan.set_shapes([(40, 40, 8), (14, 6, 3)])      # more than 1024 threads
+    with pytest.raises(RuntimeError):
+        plan.set_shapes([(26, 6, 8)])                   # one shape per level
