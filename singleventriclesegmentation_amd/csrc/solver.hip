@@ -346,30 +346,32 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   if (ring_ok)
     for (int c = 0; c < 3; ++c) ringc[c] = bld(uin, ring_vo, c * F4 + kstart * P4);
   float r2a[3] = {0.0f, 0.0f, 0.0f}, r2b[3] = {0.0f, 0.0f, 0.0f}, r2c[3];
+  // Each step reads this thread's u of the previous plane back from its own
+  // slot of the other LDS buffer (only this thread writes that slot), so no
+  // register carries it across a step; seed that slot for the first step.
+  {
+    float* Ub = U + ((kstart + 1) & 1) * 3 * UPL;
+    for (int c = 0; c < 3; ++c) Ub[c * UPL + uo] = um[c];
+  }
 
   for (int k = kstart; k <= kend; ++k) {
     const int buf = k & 1;
+    {
+      const float* Ubm = U + (buf ^ 1) * 3 * UPL;
+      for (int c = 0; c < 3; ++c) um[c] = Ubm[c * UPL + uo];
+    }
     // ---- prefetch what step k+1 needs; consumed after this step's barrier.
     // Plane indices are clamped into the volume and lanes outside it read
     // in-plane offset 0, so every load is in bounds and needs no branch: a
     // clamped or outside value only reaches steps that are skipped (k >= NZ,
     // outside z0..z1) or derivative branches the boundary rule does not select
     // (the last plane's upper neighbour, a face's outer neighbour).
+    // They are issued after the dual step, which needs the most registers
+    // (p and SN after it has consumed plane k's copies, RG after the primal
+    // step), so nothing is in flight while it runs; the primal step covers
+    // their latency.
     const uint32_t k1 = (uint32_t)min(k + 1, NZ - 1), k2 = (uint32_t)min(k + 2, NZ - 1);
     float un[3], pn[9], snn[4] = {0.0f, 0.0f, 0.0f, 0.0f}, rgn[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ringn[3];
-    for (int c = 0; c < 3; ++c) un[c] = bld(uin, vo, c * F4 + k2 * P4);
-    for (int e = 0; e < 9; ++e) pn[e] = bld(pin, vo, e * F4 + k1 * P4);
-    if (ANISO) {
-      const svs_f4 sn = bld4(SNr, vo4, 4u * k1 * P4);
-      snn[0] = sn.x; snn[1] = sn.y; snn[2] = sn.z; snn[3] = sn.w;
-    }
-    if (interior) {
-      const svs_f4 rg = bld4(RGr, vo4, 4u * (uint32_t)min(k, NZ - 1) * P4);
-      rgn[0] = rg.x; rgn[1] = rg.y; rgn[2] = rg.z; rgn[3] = rg.w;
-    }
-    for (int c = 0; c < 3; ++c) ringn[c] = 0.0f;
-    if (ring)
-      for (int c = 0; c < 3; ++c) ringn[c] = bld(uin, ring_vo, c * F4 + k1 * P4);
 
     // ---- u of plane k into LDS
     if (k < NZ) {
@@ -424,6 +426,12 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
       }
     }
 
+    for (int c = 0; c < 3; ++c) un[c] = bld(uin, vo, c * F4 + k2 * P4);
+    for (int c = 0; c < 3; ++c) ringn[c] = 0.0f;
+    if (ring)
+      for (int c = 0; c < 3; ++c) ringn[c] = bld(uin, ring_vo, c * F4 + k1 * P4);
+    for (int e = 0; e < 9; ++e) pn[e] = bld(pin, vo, e * F4 + k1 * P4);
+
     // ---- primal step at plane k-1 on the tile interior
     const int km = k - 1;
     if (interior && km >= z0 && km < z1) {
@@ -445,11 +453,18 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
       prox_primal_point(v, rho, g, factor);
       for (int c = 0; c < 3; ++c) bst(uout, vo, c * F4 + km * P4, v[c]);
     }
+    if (ANISO) {
+      const svs_f4 sn = bld4(SNr, vo4, 4u * k1 * P4);
+      snn[0] = sn.x; snn[1] = sn.y; snn[2] = sn.z; snn[3] = sn.w;
+    }
+    if (interior) {
+      const svs_f4 rg = bld4(RGr, vo4, 4u * (uint32_t)min(k, NZ - 1) * P4);
+      rgn[0] = rg.x; rgn[1] = rg.y; rgn[2] = rg.z; rgn[3] = rg.w;
+    }
 
     for (int c = 0; c < 3; ++c) {
       r2a[c] = r2b[c];
       r2b[c] = r2c[c];
-      um[c] = uc[c];
       uc[c] = up[c];
       up[c] = un[c];
       ringc[c] = ringn[c];
@@ -545,7 +560,7 @@ static std::vector<SweepShape> tile_candidates(const LevelPlan& L) {
   int maxNX = 0, maxNY = 0;
   for (auto& d : L.host) { maxNX = std::max(maxNX, d.NX); maxNY = std::max(maxNY, d.NY); }
   std::vector<int> txs = {maxNX, (maxNX + 1) / 2, (maxNX + 2) / 3, (maxNX + 3) / 4, 16, 8};
-  std::vector<int> tys = {2, 4, 6, 8, 10, 14, 20};
+  std::vector<int> tys = {2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 20};
   std::vector<SweepShape> out;
   for (int tx : txs) {
     if (tx < 4 || tx > 126) continue;
@@ -564,9 +579,13 @@ static std::vector<SweepShape> tile_candidates(const LevelPlan& L) {
 static std::vector<int> cz_candidates(const LevelPlan& L) {
   int maxNZ = 0;
   for (auto& d : L.host) maxNZ = std::max(maxNZ, d.NZ);
+  // Short chunks for small levels, and maxNZ split into 1..6 equal chunks.
   std::vector<int> out;
-  for (int cz : {1, 2, 4, 8, 16, 40, maxNZ})
-    if (cz <= maxNZ && std::find(out.begin(), out.end(), cz) == out.end()) out.push_back(cz);
+  auto add = [&](int cz) {
+    if (cz >= 1 && cz <= maxNZ && std::find(out.begin(), out.end(), cz) == out.end()) out.push_back(cz);
+  };
+  for (int cz : {1, 2, 4, 8}) add(cz);
+  for (int parts = 1; parts <= 6; ++parts) add((maxNZ + parts - 1) / parts);
   return out;
 }
 
