@@ -1104,19 +1104,28 @@ int svs_tvl1_plan_autotune(svs_plan* pl, int reps, void* stream, int* report /* 
       SVS_HIP(hipEventElapsedTime(ms, e0, e1));
       return 0;
     };
+    // Stage 1: every tile with its default z-chunk.  Stage 2: every z-chunk
+    // for the three fastest tiles of stage 1.
     SweepShape best{L.bx, L.by, L.cz};
     float best_ms = 0.0f;
     if ((rc = time_shape(best.bx, best.by, best.cz, &best_ms))) break;
+    std::vector<std::pair<float, SweepShape>> ranked = {{best_ms, best}};
     for (const SweepShape& c : tile_candidates(L)) {
       float ms = 0.0f;
       if ((rc = time_shape(c.bx, c.by, c.cz, &ms))) break;
+      ranked.push_back({ms, c});
       if (ms < best_ms) { best_ms = ms; best = c; }
     }
     if (rc) break;
-    for (int cz : cz_candidates(L)) {
-      float ms = 0.0f;
-      if ((rc = time_shape(best.bx, best.by, cz, &ms))) break;
-      if (ms < best_ms) { best_ms = ms; best.cz = cz; }
+    std::sort(ranked.begin(), ranked.end(),
+              [](const std::pair<float, SweepShape>& a, const std::pair<float, SweepShape>& b) { return a.first < b.first; });
+    for (size_t r = 0; r < ranked.size() && r < 3 && !rc; ++r) {
+      const SweepShape t = ranked[r].second;
+      for (int cz : cz_candidates(L)) {
+        float ms = 0.0f;
+        if ((rc = time_shape(t.bx, t.by, cz, &ms))) break;
+        if (ms < best_ms) { best_ms = ms; best = SweepShape{t.bx, t.by, cz}; }
+      }
     }
     if (rc) break;
     L.bx = best.bx; L.by = best.by; L.cz = best.cz;
