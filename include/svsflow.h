@@ -115,6 +115,26 @@ int svs_blur3d(const float* in, int NZ, int NY, int NX, const float* weights, in
  * svs/modules/flow/processor.py:116-117.  `out` must not alias `in`. */
 int svs_median3d(const float* in, int B, int NZ, int NY, int NX, int C, int ks, float* out, void* stream);
 
+/* ------------------------------------------------ mask metrics (warp harness) */
+
+/* |A|, |B| and |A and B| of two binary masks [Z,Y,X] (value > 0.5 is set) into
+ * counts[0..2] (device, 64-bit).  Dice = 2|A and B| / (|A| + |B|): monai's
+ * compute_dice as svs/modules/flow/warping.py:264,267 calls it. */
+int svs_mask_counts(const float* a, const float* b, int NZ, int NY, int NX, unsigned long long* counts,
+                    void* stream);
+
+/* Workspace bytes of svs_mask_hausdorff2 for one volume size. */
+size_t svs_mask_hausdorff_workspace_bytes(int NZ, int NY, int NX);
+
+/* Directed squared Hausdorff distances between the edge voxels (6-connected
+ * boundary: mask XOR its binary erosion) of two masks: out[0] = A -> B,
+ * out[1] = B -> A (device, exact integers in voxel units).  The workspace's
+ * first two 64-bit words receive the edge counts of A and B.  monai's
+ * compute_hausdorff_distance (svs/modules/flow/warping.py:265,268) is the
+ * square root of the larger one. */
+int svs_mask_hausdorff2(const float* a, const float* b, int NZ, int NY, int NX, void* workspace, size_t ws_bytes,
+                        unsigned long long* out, void* stream);
+
 /* ---------------------------------------------------- the whole solve */
 
 /* Constructor arguments of TVL13DOpticalFlow (svs/modules/flow/tvl13d.py:14-67)

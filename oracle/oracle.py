@@ -208,6 +208,51 @@ def median3d(u, ks, dtype=np.float32):
     return out if batched else out[0]
 
 
+def mask_counts(a, b):
+    """(|A|, |B|, |A and B|) of two binary masks (value > 0.5 is set)."""
+    a = _arr(a, np.float32)
+    b = _arr(b, np.float32)
+    out = np.zeros(3, np.int64)
+    lib().orc_mask_counts(_p(a), _p(b), ctypes.c_longlong(a.size), _p(out))
+    return tuple(int(v) for v in out)
+
+
+def dice(a, b):
+    """monai compute_dice on one binary channel: 2|A and B| / (|A| + |B|) in float32,
+    NaN when the ground truth ``b`` is empty (ignore_empty=True)."""
+    na, nb, nab = mask_counts(a, b)
+    if nb == 0:
+        return float("nan")
+    return float(np.float32(2.0 * nab) / np.float32(na + nb))
+
+
+def hausdorff2(a, b):
+    """Directed squared Hausdorff distances (A->B, B->A) between the 6-connected
+    edge voxels of two [Z,Y,X] masks; -1: the source has no edge, -2: only the
+    target has none."""
+    a = _arr(a, np.float32)
+    b = _arr(b, np.float32)
+    out = np.zeros(2, np.int64)
+    lib().orc_mask_hausdorff2(_p(a), _p(b), *a.shape, _p(out))
+    return int(out[0]), int(out[1])
+
+
+def hausdorff(a, b):
+    """Symmetric Hausdorff distance as monai returns it (float32 of the float64
+    root); NaN when either mask has no edge voxel."""
+    d = hausdorff2(a, b)
+    if min(d) < 0:
+        return float("nan")
+    return float(np.float32(np.sqrt(np.float64(max(d)))))
+
+
+def edge_count(m):
+    m = _arr(m, np.float32)
+    fn = lib().orc_mask_edge_count
+    fn.restype = ctypes.c_longlong
+    return int(fn(_p(m), *m.shape))
+
+
 def solve_params(num_scales=4, max_warps=10, max_outer_iterations=50, weight_matching=1.0, weight_tv=0.025,
                  pd_type=2, pd_params=(0.5, 0.5, 0.25, 1.0), use_aniso=True, aniso_params=(5.0, 0.5),
                  interpolation="CUBIC_HERMITESPLINE", boundary="MIRROR", length_type="numDofs",

@@ -24,6 +24,8 @@
 //   warp adjoint ............. dot-product identity (image), FP64 central FD (flow)
 //   blur ..................... torch.nn.functional.conv3d on CPU
 //   median filter ............ scipy.ndimage.median_filter on CPU
+//   mask metrics ............. scipy.ndimage binary_erosion + distance_transform_edt
+//                              (what monai's Dice/Hausdorff call), closed forms
 //   solve .................... FP64 twin, EPE against a known displacement
 #include <cmath>
 #include <cstdint>
@@ -679,6 +681,56 @@ static void median3d(const T* in, int B, int NZ, int NY, int NX, int C, int ks, 
     }
 }
 
+// ------------------------------------------------------- mask metrics
+// warping.py:245-270 scores propagated masks with monai's compute_dice and
+// compute_hausdorff_distance (include_background=False on one channel, which
+// keeps the channel).  Masks are binary; a voxel is set when its value > 0.5.
+//   counts: |A|, |B|, |A and B|  (Dice = 2|A and B| / (|A| + |B|)).
+//   edges:  set voxels with a 6-neighbour that is unset or outside the volume
+//           (mask XOR binary_erosion(mask) with scipy's default cross).
+//   directed Hausdorff A -> B: the largest, over edge voxels of A, of the
+//   squared Euclidean distance to the nearest edge voxel of B (voxel units);
+//   -1 when A has no edge voxel, -2 when only B has none.
+static void mask_counts(const float* a, const float* b, size_t n, long long* out) {
+  long long na = 0, nb = 0, nab = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const bool x = a[i] > 0.5f, y = b[i] > 0.5f;
+    na += x; nb += y; nab += (x && y);
+  }
+  out[0] = na; out[1] = nb; out[2] = nab;
+}
+
+static std::vector<int> mask_edges(const float* m, int NZ, int NY, int NX) {
+  std::vector<int> out;
+  auto set = [&](int z, int y, int x) {
+    return z >= 0 && z < NZ && y >= 0 && y < NY && x >= 0 && x < NX && m[((size_t)z * NY + y) * NX + x] > 0.5f;
+  };
+  for (int z = 0; z < NZ; ++z)
+    for (int y = 0; y < NY; ++y)
+      for (int x = 0; x < NX; ++x)
+        if (set(z, y, x) && !(set(z - 1, y, x) && set(z + 1, y, x) && set(z, y - 1, x) && set(z, y + 1, x) &&
+                              set(z, y, x - 1) && set(z, y, x + 1))) {
+          out.push_back(z); out.push_back(y); out.push_back(x);
+        }
+  return out;
+}
+
+static long long directed_hausdorff2(const std::vector<int>& ea, const std::vector<int>& eb) {
+  if (ea.empty()) return -1;
+  if (eb.empty()) return -2;
+  long long worst = 0;
+  for (size_t i = 0; i < ea.size(); i += 3) {
+    long long best = -1;
+    for (size_t j = 0; j < eb.size(); j += 3) {
+      const long long dz = ea[i] - eb[j], dy = ea[i + 1] - eb[j + 1], dx = ea[i + 2] - eb[j + 2];
+      const long long d = dz * dz + dy * dy + dx * dx;
+      if (best < 0 || d < best) best = d;
+    }
+    worst = std::max(worst, best);
+  }
+  return worst;
+}
+
 // TVL13DOpticalFlow.compute, tvl13d.py:69-162.  I0/I1 are already blurred
 // when the caller wants the blur (the blur weights come from flow_utils.py).
 template <class T> static void solve(const SolveParams& P, const T* I0, const T* I1, int NZ, int NY, int NX,
@@ -795,6 +847,18 @@ extern "C" float orc_svs_expf(float x) { return svs_expf(x); }
 extern "C" float orc_svs_powf(float x, float y) { return svs_powf(x, y); }
 extern "C" double orc_svs_exp_d(double x) { return svs_exp_d(x); }
 extern "C" double orc_svs_log_d(double x) { return svs_log_d(x); }
+
+extern "C" void orc_mask_counts(const float* a, const float* b, long long n, long long* out) {
+  mask_counts(a, b, (size_t)n, out);
+}
+extern "C" void orc_mask_hausdorff2(const float* a, const float* b, int NZ, int NY, int NX, long long* out) {
+  const std::vector<int> ea = mask_edges(a, NZ, NY, NX), eb = mask_edges(b, NZ, NY, NX);
+  out[0] = directed_hausdorff2(ea, eb);
+  out[1] = directed_hausdorff2(eb, ea);
+}
+extern "C" long long orc_mask_edge_count(const float* m, int NZ, int NY, int NX) {
+  return (long long)(mask_edges(m, NZ, NY, NX).size() / 3);
+}
 
 extern "C" void orc_step_schedule(const double* params, double* sig, double* tau) {
   SolveParams P;

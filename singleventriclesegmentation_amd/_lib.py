@@ -52,6 +52,9 @@ _SIGNATURES = {
     "svs_prolongate3d": ([_vp, _i, _i, _i, _i, _f, _f, _f, _i, _i, _i, _f, _f, _f, _i, _i, _vp, _vp], _i),
     "svs_blur3d": ([_vp, _i, _i, _i, _vp, _i, _vp, _vp], _i),
     "svs_median3d": ([_vp, _i, _i, _i, _i, _i, _i, _vp, _vp], _i),
+    "svs_mask_counts": ([_vp, _vp, _i, _i, _i, _vp, _vp], _i),
+    "svs_mask_hausdorff_workspace_bytes": ([_i, _i, _i], ctypes.c_size_t),
+    "svs_mask_hausdorff2": ([_vp, _vp, _i, _i, _i, _vp, ctypes.c_size_t, _vp, _vp], _i),
     "svs_tvl1_workspace_bytes": ([ctypes.POINTER(TVL1Params), _i, ctypes.POINTER(_i)], ctypes.c_size_t),
     "svs_tvl1_plan_create": ([ctypes.POINTER(TVL1Params), _i, ctypes.POINTER(_i), _vp, ctypes.c_size_t, _vp,
                               ctypes.POINTER(_vp)], _i),

@@ -203,3 +203,23 @@ def test_gpu_reproduces_golden_ops(dev):
 
 def g_(a, dev):
     return g(a, dev)
+
+
+@pytest.mark.parametrize("src,dst,T,pad", [((9, 40, 36), (20, 24, 22), 7, 10), ((12, 160, 144), (80, 80, 80), 25, 35)])
+def test_preprocessing_resize_matches_oracle(O, dev, src, dst, T, pad):
+    """svs/preprocessing/prolongation.py:69-85 with conf/preprocessing.yaml (LINEAR,
+    MIRROR, numDofs): the 4D image through forwardVectorField, the masks through
+    forward + round, zero frames up to pad_time."""
+    from singleventriclesegmentation_amd.preprocessing import prolongate_patient
+    r = np.random.default_rng(sum(src) + T)
+    img = r.random(src + (T,)).astype(np.float32)
+    m_dia = (r.random(src) > 0.6).astype(np.float32)
+    m_sys = (r.random(src) > 0.4).astype(np.float32)
+    out, a, b = prolongate_patient(g(img, dev), g(m_dia, dev), g(m_sys, dev), shape_zyx=dst, pad_time=pad)
+    Ls = tuple(n - 1 for n in src)
+    Ld = tuple(n - 1 for n in dst)
+    assert tuple(out.shape) == dst + (pad,)
+    assert np.array_equal(h(out)[..., :T], O.prolongate(img, Ls, dst, Ld, ip=1, bc=1))
+    assert not h(out)[..., T:].any()
+    assert np.array_equal(h(a), np.round(O.prolongate(m_dia, Ls, dst, Ld, ip=1, bc=1)))
+    assert np.array_equal(h(b), np.round(O.prolongate(m_sys, Ls, dst, Ld, ip=1, bc=1)))
