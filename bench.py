@@ -136,7 +136,7 @@ def main():
         log(args, "fixed sweep shapes (bx, by, cz) per level:", tuned)
     else:
         log(args, "warm-up and sweep-shape autotuning")
-        tuned = plan.autotune()
+        tuned = plan.autotune(reps=5)
         log(args, "tuned sweep shapes (bx, by, cz) per level:", tuned)
     for _ in range(args.warmup):
         step()  # the first step after tuning re-captures the graph

@@ -1105,7 +1105,8 @@ int svs_tvl1_plan_autotune(svs_plan* pl, int reps, void* stream, int* report /* 
       return 0;
     };
     // Stage 1: every tile with its default z-chunk.  Stage 2: every z-chunk
-    // for the three fastest tiles of stage 1.
+    // for the five fastest tiles of stage 1 (shapes a few per cent apart
+    // change rank between repetitions).
     SweepShape best{L.bx, L.by, L.cz};
     float best_ms = 0.0f;
     if ((rc = time_shape(best.bx, best.by, best.cz, &best_ms))) break;
@@ -1119,7 +1120,7 @@ int svs_tvl1_plan_autotune(svs_plan* pl, int reps, void* stream, int* report /* 
     if (rc) break;
     std::sort(ranked.begin(), ranked.end(),
               [](const std::pair<float, SweepShape>& a, const std::pair<float, SweepShape>& b) { return a.first < b.first; });
-    for (size_t r = 0; r < ranked.size() && r < 3 && !rc; ++r) {
+    for (size_t r = 0; r < ranked.size() && r < 5 && !rc; ++r) {
       const SweepShape t = ranked[r].second;
       for (int cz : cz_candidates(L)) {
         float ms = 0.0f;
