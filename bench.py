@@ -339,6 +339,8 @@ def bench_serve(args, dev):
     log(args, "building request pool")
     pool = make_pool(sizes, args.frames, dev)
     server = FlowServer(cfg, device=dev, hbm_budget_bytes=64 << 30, slots=4)
+    log(args, "warm-up: creating and tuning the plans of every size and bucket")
+    server.warm({sh: (pool[sh][0][1], pool[sh][0][0]) for sh in sizes}, frames=args.frames)
     warm = poisson_requests(0, 6, sizes, pool, args.frames, seed=99)
     run_service(server, warm)
     res = {}

@@ -126,6 +126,36 @@ class FlowServer:
             evictable -= next(p.workspace_bytes for k, p in self.idle_plans if k == key)
         return self.used - evictable + need <= self.budget
 
+    # ------------------------------------------------------------ warm-up
+    def warm(self, pairs, frames: Optional[int] = None, autotune: bool = True, reps: int = 3):
+        """Creates the plans a workload will need before requests arrive: every
+        pair bucket and, with ``frames``, the 2(frames-1)-instance sequence plan,
+        for each volume size in ``pairs`` (a dict size -> (I0, I1) of real
+        frames, so that the timed sweeps see the data's branch pattern).  With
+        ``autotune`` each plan's sweep shapes are tuned once; plans of the same
+        geometry created later in the process start from those shapes.  Plans
+        that do not fit the HBM budget are skipped."""
+        keys = [(tuple(sh), b) for sh in pairs for b in self.pair_buckets]
+        if frames:
+            keys += [(tuple(sh), 2 * (frames - 1)) for sh in pairs]
+        for key in keys:
+            if any(k == key for k, _ in self.idle_plans):
+                continue
+            need = self._ws(key)
+            self._make_room(need)
+            if self.used + need > self.budget:
+                continue
+            plan = Plan(self.cfg, [key[0]] * key[1], device=self.device)
+            self.used += plan.workspace_bytes
+            i0, i1 = pairs[key[0]]
+            for k in range(key[1]):
+                plan.bind(k, i0, i1)
+            plan.run(use_graph=False)
+            if autotune:
+                plan.autotune(reps=reps)
+            self.idle_plans.append((key, plan))
+        torch.cuda.synchronize(self.device)
+
     # ---------------------------------------------------------- requests
     def submit(self, req: Request):
         self.pending.append(req)

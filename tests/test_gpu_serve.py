@@ -60,3 +60,23 @@ def test_poisson_arrivals_complete(dev):
     print(f"10 requests at 200/s: p50 {np.percentile(lat, 50) * 1e3:.1f} ms, p99 {np.percentile(lat, 99) * 1e3:.1f} ms")
     assert all(r.done is not None and r.done >= r.arrival for r in reqs)
     server.close()
+
+
+def test_warmed_and_tuned_plans_keep_results(dev):
+    """warm() creates and autotunes the plans of every size and bucket up front;
+    requests served from those plans still equal the plain drop-in alone, and
+    warm() stays inside the HBM budget."""
+    cfg = SolverConfig(num_scales=2, max_warps=2, max_outer_iterations=4)
+    pool = make_pool(SHAPES, FRAMES, dev)
+    server = FlowServer(cfg, device=dev, slots=2, pair_buckets=(1, 2, 4))
+    server.warm({sh: (pool[sh][0][1], pool[sh][0][0]) for sh in SHAPES}, frames=FRAMES, reps=2)
+    assert len(server.idle_plans) == len(SHAPES) * 4 and server.used <= server.budget
+    reqs = poisson_requests(0, 8, SHAPES, pool, FRAMES, seed=8, p_pair=0.75, mask_last=3)
+    run_service(server, reqs)
+    check_against_alone(reqs, dev)
+    server.close()
+    small = FlowServer(cfg, device=dev, hbm_budget_bytes=int(1.5 * workspace_bytes(cfg, [SHAPES[2]] * 4)),
+                       pair_buckets=(1, 2, 4))
+    small.warm({sh: (pool[sh][0][1], pool[sh][0][0]) for sh in SHAPES}, frames=FRAMES, autotune=False)
+    assert small.used <= small.budget
+    small.close()
