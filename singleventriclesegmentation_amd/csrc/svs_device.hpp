@@ -46,6 +46,26 @@ __device__ __forceinline__ float cr_local(float b0, float b1, float b2, float b3
   return w * (w * (w * a + b) + c) + d;
 }
 
+// cr_local for lc < 2 (the usual case): the same operations without the
+// selects of the rounded-up segment, hence the same bits whenever lc < 2.
+__device__ __forceinline__ float cr_local_lo(float b0, float b1, float b2, float b3, float lc) {
+  const float p_f = b1;
+  const float pp_f = (b2 - b0) / 2;
+  const float p_c = b2;
+  const float pp_c = (b3 - b1) / 2;
+  const float a = 2 * p_f - 2 * p_c + pp_f + pp_c;
+  const float b = -3 * p_f + 3 * p_c - 2 * pp_f - pp_c;
+  const float c = pp_f;
+  const float d = p_f;
+  const float w = lc - 1.0f;
+  return w * (w * (w * a + b) + c) + d;
+}
+
+template <bool MAYBE_HI>
+__device__ __forceinline__ float cr_eval(float b0, float b1, float b2, float b3, float lc) {
+  return MAYBE_HI ? cr_local(b0, b1, b2, b3, lc) : cr_local_lo(b0, b1, b2, b3, lc);
+}
+
 // Adjoint of cr_local (interpolation_cubicHermiteSpline.cuh:59-117):
 // g[k] = d out/d b[k] * fwd, returns d out/d w * fwd.  In the rounded-up case
 // the tap that falls outside the 4-entry buffer contributes nothing and b[0]
@@ -97,9 +117,9 @@ __device__ __forceinline__ Axis make_axis(float coord, float h) {
 // y, x, v) loads the NC values at an in-range voxel.  Loop order follows
 // interpolation_cubicHermiteSpline.cuh:482-525: x-pass per (dz,dy), y-pass per
 // dz, then the z-pass; each field is evaluated exactly as the scalar routine.
-template <int NC, class Fetch>
-__device__ __forceinline__ void sample_cubic(const Fetch& fetch, int NZ, int NY, int NX, Axis az, Axis ay, Axis ax,
-                                             int bc, float out[NC]) {
+template <int NC, bool MAYBE_HI, class Fetch>
+__device__ __forceinline__ void sample_cubic_impl(const Fetch& fetch, int NZ, int NY, int NX, Axis az, Axis ay,
+                                                  Axis ax, int bc, float out[NC]) {
   int xs[4], ys[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -118,13 +138,25 @@ __device__ __forceinline__ void sample_cubic(const Fetch& fetch, int NZ, int NY,
 #pragma unroll
       for (int dx = 0; dx < 4; ++dx) fetch(z, ys[dy], xs[dx], v[dx]);
 #pragma unroll
-      for (int c = 0; c < NC; ++c) by[c][dy] = cr_local(v[0][c], v[1][c], v[2][c], v[3][c], lx);
+      for (int c = 0; c < NC; ++c) by[c][dy] = cr_eval<MAYBE_HI>(v[0][c], v[1][c], v[2][c], v[3][c], lx);
     }
 #pragma unroll
-    for (int c = 0; c < NC; ++c) bz[c][dz] = cr_local(by[c][0], by[c][1], by[c][2], by[c][3], ly);
+    for (int c = 0; c < NC; ++c) bz[c][dz] = cr_eval<MAYBE_HI>(by[c][0], by[c][1], by[c][2], by[c][3], ly);
   }
 #pragma unroll
-  for (int c = 0; c < NC; ++c) out[c] = cr_local(bz[c][0], bz[c][1], bz[c][2], bz[c][3], lz);
+  for (int c = 0; c < NC; ++c) out[c] = cr_eval<MAYBE_HI>(bz[c][0], bz[c][1], bz[c][2], bz[c][3], lz);
+}
+
+// The rounded-up segment (w + 1 == 2.0f, i.e. w = 1 - 2^-24) is rare: a
+// wavefront in which no lane meets it takes the select-free evaluation.
+template <int NC, class Fetch>
+__device__ __forceinline__ void sample_cubic(const Fetch& fetch, int NZ, int NY, int NX, Axis az, Axis ay, Axis ax,
+                                             int bc, float out[NC]) {
+  const bool hi = ax.w + 1 >= 2.0f || ay.w + 1 >= 2.0f || az.w + 1 >= 2.0f;
+  if (__any(hi))
+    sample_cubic_impl<NC, true>(fetch, NZ, NY, NX, az, ay, ax, bc, out);
+  else
+    sample_cubic_impl<NC, false>(fetch, NZ, NY, NX, az, ay, ax, bc, out);
 }
 
 // interpolation_linear.cuh:249-295.
