@@ -257,8 +257,13 @@ __global__ __launch_bounds__(kBlock) void k_store_output(const LevelDesc* __rest
 // derivatives of the ring) and the x/y derivative columns of r for planes
 // k-1 and k; z-neighbours live in registers of the owning thread.  One
 // barrier per plane.
+// A work-group's tile: instance, origin, z-range and its own shape, so that
+// one launch can give every instance of an uneven batch full-row tiles.
 struct alignas(16) SweepTile {
-  int inst, x0, y0, zr;  // zr = z0 | (z1 << 16)
+  int inst;
+  int xy;     // x0 | (y0 << 16): first interior voxel
+  int zr;     // z0 | (z1 << 16)
+  int shape;  // bx | (by << 16): work-group extent of this tile, ring included
 };
 
 // UNIT_H: every instance of the launch has h = 1 on all three axes (always
@@ -274,19 +279,23 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   const int NZ = d.NZ, NY = d.NY, NX = d.NX;
   const int64_t N = d.N;
   const float hZ = UNIT_H ? 1.0f : d.hZ, hY = UNIT_H ? 1.0f : d.hY, hX = UNIT_H ? 1.0f : d.hX;
-  const int BX = blockDim.x, BY = blockDim.y;
-  const int ex = threadIdx.x, ey = threadIdx.y;
-  const int x = t.x0 - 1 + ex, y = t.y0 - 1 + ey;
+  // The work-group is one-dimensional and sized for the largest tile of the
+  // launch; threads past this tile's BX * BY only take part in the barriers.
+  const int BX = t.shape & 0xffff, BY = t.shape >> 16;
+  const int tid = threadIdx.x;
+  const bool active = tid < BX * BY;
+  const int ex = tid % BX, ey = tid / BX;
+  const int x = (t.xy & 0xffff) - 1 + ex, y = (t.xy >> 16) - 1 + ey;
   const int z0 = t.zr & 0xffff, z1 = t.zr >> 16;
-  const bool valid = x >= 0 && x < NX && y >= 0 && y < NY;
+  const bool valid = active && x >= 0 && x < NX && y >= 0 && y < NY;
   const bool interior = valid && ex >= 1 && ex <= BX - 2 && ey >= 1 && ey <= BY - 2;
 
-  const int UW = BX + 2, UH = BY + 2, UPL = UW * UH, RPL = BX * BY;
+  const int UW = BX + 2, UH = BY + 2, UPL = UW * UH;
   float* U = lds;                 // [2][3][UPL]
-  float* R = lds + 6 * UPL;       // [2][6][RPL]
+  float* R = lds + 6 * UPL;       // [2][6][BX * BY]
+  const int RPL = BX * BY;
   const int uo = (ey + 1) * UW + (ex + 1);
-  const int ro = ey * BX + ex;
-  const int tid = ey * BX + ex;
+  const int ro = tid;
 
   // Buffer addressing: this thread's in-plane byte offset in a VGPR, the
   // field and plane offsets in SGPRs (the host keeps 36*N < 2^32).
@@ -315,7 +324,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
       uy = 1 + (rr >> 1);
       ux = (rr & 1) ? UW - 1 : 0;
     }
-    const int gx = t.x0 - 2 + ux, gy = t.y0 - 2 + uy;
+    const int gx = (t.xy & 0xffff) - 2 + ux, gy = (t.xy >> 16) - 2 + uy;
     ring_ok = gx >= 0 && gx < NX && gy >= 0 && gy < NY;
     ring_vo = ring_ok ? (uint32_t)(gy * NX + gx) * 4u : 0u;
     ring_lds = uy * UW + ux;
@@ -349,14 +358,14 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   // Each step reads this thread's u of the previous plane back from its own
   // slot of the other LDS buffer (only this thread writes that slot), so no
   // register carries it across a step; seed that slot for the first step.
-  {
+  if (active) {
     float* Ub = U + ((kstart + 1) & 1) * 3 * UPL;
     for (int c = 0; c < 3; ++c) Ub[c * UPL + uo] = um[c];
   }
 
   for (int k = kstart; k <= kend; ++k) {
     const int buf = k & 1;
-    {
+    if (active) {
       const float* Ubm = U + (buf ^ 1) * 3 * UPL;
       for (int c = 0; c < 3; ++c) um[c] = Ubm[c * UPL + uo];
     }
@@ -374,7 +383,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
     float un[3], pn[9], snn[4] = {0.0f, 0.0f, 0.0f, 0.0f}, rgn[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ringn[3];
 
     // ---- u of plane k into LDS
-    if (k < NZ) {
+    if (k < NZ && active) {
       float* Ub = U + buf * 3 * UPL;
       for (int c = 0; c < 3; ++c) Ub[c * UPL + uo] = uc[c];
       if (ring)
@@ -420,8 +429,10 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
       }
     } else {
       for (int c = 0; c < 3; ++c) {
-        Rb[(2 * c) * RPL + ro] = 0.0f;
-        Rb[(2 * c + 1) * RPL + ro] = 0.0f;
+        if (active) {
+          Rb[(2 * c) * RPL + ro] = 0.0f;
+          Rb[(2 * c + 1) * RPL + ro] = 0.0f;
+        }
         r2c[c] = 0.0f;
       }
     }
@@ -483,7 +494,9 @@ struct LevelPlan {
   SweepTile* tiles = nullptr;
   int ntiles = 0;
   int tile_capacity = 0;
-  int bx = 0, by = 0, cz = 0;
+  int bx = 0, by = 0, cz = 0;    // shape policy (see inst_tile)
+  int threads = 0;               // work-group size: the largest tile's bx * by
+  size_t lds = 0;                // dynamic LDS of the largest tile
   bool unit_h = false;           // h == 1 on every axis of every instance
 };
 
@@ -526,29 +539,55 @@ static size_t sweep_lds_bytes(int bx, int by) {
   return sizeof(float) * (6 * (size_t)(bx + 2) * (by + 2) + 12 * (size_t)bx * by);
 }
 
+// A level's sweep shape (bx, by, cz) is a policy: tiles are at most bx - 2
+// voxels wide and use at most bx * by threads; each instance gets its own tile
+// from it (inst_tile), so volumes of different widths in one launch all march
+// full (or evenly split) rows.  For NX == bx - 2 the tile is exactly bx x by.
 struct SweepShape {
   int bx, by, cz;
 };
 
+static void inst_tile(const LevelDesc& d, int bx, int by, int& tbx, int& tby) {
+  const int txmax = bx - 2;
+  const int ntx = (d.NX + txmax - 1) / txmax;
+  const int tx = (d.NX + ntx - 1) / ntx;  // rows split into equal pieces
+  tbx = tx + 2;
+  const int ty = std::min((bx * by) / tbx - 2, d.NY);
+  tby = ty + 2;
+}
+
 static int count_tiles(const LevelPlan& L, int bx, int by, int cz) {
-  const int tx = bx - 2, ty = by - 2;
   int n = 0;
-  for (auto& d : L.host) n += ((d.NX + tx - 1) / tx) * ((d.NY + ty - 1) / ty) * ((d.NZ + cz - 1) / cz);
+  for (auto& d : L.host) {
+    int tbx, tby;
+    inst_tile(d, bx, by, tbx, tby);
+    n += ((d.NX + tbx - 3) / (tbx - 2)) * ((d.NY + tby - 3) / (tby - 2)) * ((d.NZ + cz - 1) / cz);
+  }
   return n;
 }
 
-static bool shape_ok(int bx, int by, int cz) {
-  return bx >= 3 && by >= 3 && cz >= 1 && bx * by <= 1024 && 2 * (bx + 2) + 2 * by <= bx * by &&
-         sweep_lds_bytes(bx, by) <= 96 * 1024;
+// Every instance's tile has a row or more, one thread per outer-ring
+// position and fits the LDS limit.
+static bool shape_ok(const LevelPlan& L, int bx, int by, int cz) {
+  if (bx < 3 || by < 3 || cz < 1 || bx * by > 1024) return false;
+  for (auto& d : L.host) {
+    int tbx, tby;
+    inst_tile(d, bx, by, tbx, tby);
+    if (tby < 3 || 2 * (tbx + 2) + 2 * tby > tbx * tby || sweep_lds_bytes(tbx, tby) > 96 * 1024) return false;
+  }
+  return true;
 }
 
 // z-chunk that gives the launch about `target` work-groups.
 static int default_cz(const LevelPlan& L, int bx, int by, int target, int minz) {
   int maxNZ = 0;
   for (auto& d : L.host) maxNZ = std::max(maxNZ, d.NZ);
-  const int tx = bx - 2, ty = by - 2;
   int64_t cols = 0;
-  for (auto& d : L.host) cols += (int64_t)((d.NX + tx - 1) / tx) * ((d.NY + ty - 1) / ty);
+  for (auto& d : L.host) {
+    int tbx, tby;
+    inst_tile(d, bx, by, tbx, tby);
+    cols += (int64_t)((d.NX + tbx - 3) / (tbx - 2)) * ((d.NY + tby - 3) / (tby - 2));
+  }
   int cz = maxNZ;
   while (cz > minz && cols * ((maxNZ + cz - 1) / cz) < target) cz = (cz + 1) / 2;
   return cz;
@@ -563,13 +602,13 @@ static std::vector<SweepShape> tile_candidates(const LevelPlan& L) {
   std::vector<int> tys = {2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 20};
   std::vector<SweepShape> out;
   for (int tx : txs) {
-    if (tx < 4 || tx > 126) continue;
+    if (tx < 4) continue;
     for (int ty : tys) {
       if (ty > maxNY && ty != 2) continue;
       const int bx = tx + 2, by = ty + 2;
       bool dup = false;
       for (auto& c : out) dup |= (c.bx == bx && c.by == by);
-      if (dup || !shape_ok(bx, by, 1)) continue;
+      if (dup || !shape_ok(L, bx, by, 1)) continue;
       out.push_back({bx, by, default_cz(L, bx, by, 2048, 2)});
     }
   }
@@ -643,14 +682,14 @@ static void choose_sweep_shape(const svs_tvl1_params& P, LevelPlan& L, int ninst
     // one outer-ring position per thread
     while (2 * (bx + 2) + 2 * by > bx * by) ++by;
   }
-  if (cz <= 0) {
-    const int tx = bx - 2, ty = by - 2;
-    int64_t cols = 0;
-    for (auto& d : L.host) cols += (int64_t)((d.NX + tx - 1) / tx) * ((d.NY + ty - 1) / ty);
-    const int target = big ? 1024 : 2048, minz = big ? 40 : (tiny ? 2 : 4);
-    cz = maxNZ;
-    while (cz > minz && cols * ((maxNZ + cz - 1) / cz) < target) cz = (cz + 1) / 2;
+  if (!shape_ok(L, bx, by, 1)) {
+    // The policy fits the widest volume; a narrow one may need a taller tile.
+    int b = by;
+    while (b * bx <= 1024 && !shape_ok(L, bx, b, 1)) ++b;
+    if (shape_ok(L, bx, b, 1)) by = b;
+    else { bx = 12; by = 12; }
   }
+  if (cz <= 0) cz = default_cz(L, bx, by, big ? 1024 : 2048, big ? 40 : (tiny ? 2 : 4));
   L.bx = bx; L.by = by; L.cz = cz;
 }
 
@@ -738,7 +777,7 @@ static int validate_dims(const svs_tvl1_params* P, int ninst, const int* dims) {
   for (int k = 0; k < ninst; ++k) {
     int nz = dims[3 * k], ny = dims[3 * k + 1], nx = dims[3 * k + 2];
     SVS_REQUIRE_DIMS(nz, ny, nx);
-    SVS_REQUIRE(nz < 32768, "NZ must be < 32768");
+    SVS_REQUIRE(nz < 32768 && ny < 32768 && nx < 32768, "every dimension must be < 32768");
     // The fused sweep addresses the 9 dual fields of an instance with 32-bit
     // byte offsets.
     SVS_REQUIRE((int64_t)nz * ny * nx <= (int64_t)0xFFFFFFFF / 36,
@@ -821,14 +860,20 @@ static int upload_tiles(svs_plan* pl, int s, hipStream_t st) {
     SVS_REQUIRE(L.ntiles <= L.tile_capacity, "tile table overflow (%d > %d)", L.ntiles, L.tile_capacity);
     std::vector<SweepTile> natural;
     natural.reserve(L.ntiles);
-    const int tx = L.bx - 2, ty = L.by - 2;
+    L.threads = 0;
+    L.lds = 0;
     for (int k = 0; k < pl->ninst; ++k) {
       const LevelDesc& d = L.host[k];
+      int tbx, tby;
+      inst_tile(d, L.bx, L.by, tbx, tby);
+      L.threads = std::max(L.threads, tbx * tby);
+      L.lds = std::max(L.lds, sweep_lds_bytes(tbx, tby));
+      const int tx = tbx - 2, ty = tby - 2;
       for (int z0 = 0; z0 < d.NZ; z0 += L.cz)
         for (int y0 = 0; y0 < d.NY; y0 += ty)
           for (int x0 = 0; x0 < d.NX; x0 += tx) {
             const int z1 = std::min(d.NZ, z0 + L.cz);
-            natural.push_back(SweepTile{k, x0, y0, z0 | (z1 << 16)});
+            natural.push_back(SweepTile{k, x0 | (y0 << 16), z0 | (z1 << 16), tbx | (tby << 16)});
           }
     }
     // Work-group b runs on XCD b % 8.  Deal contiguous runs of the natural
@@ -854,12 +899,11 @@ static int upload_tiles(svs_plan* pl, int s, hipStream_t st) {
 template <int BC, bool ANISO>
 static void launch_sweep_t(const LevelPlan& L, int parity, float sg, float ta, float fac, float wtv, int wz,
                            hipStream_t st) {
-  const size_t lds = sweep_lds_bytes(L.bx, L.by);
   if (L.unit_h)
-    hipLaunchKernelGGL((k_sweep<BC, ANISO, true>), dim3(L.ntiles), dim3(L.bx, L.by), lds, st, L.dev, L.tiles,
+    hipLaunchKernelGGL((k_sweep<BC, ANISO, true>), dim3(L.ntiles), dim3(L.threads), L.lds, st, L.dev, L.tiles,
                        parity, sg, ta, fac, wtv, wz);
   else
-    hipLaunchKernelGGL((k_sweep<BC, ANISO, false>), dim3(L.ntiles), dim3(L.bx, L.by), lds, st, L.dev, L.tiles,
+    hipLaunchKernelGGL((k_sweep<BC, ANISO, false>), dim3(L.ntiles), dim3(L.threads), L.lds, st, L.dev, L.tiles,
                        parity, sg, ta, fac, wtv, wz);
 }
 
@@ -881,10 +925,9 @@ static void launch_sweep(const svs_plan* pl, const LevelPlan& L, int parity, int
 
 // Dynamic LDS above 64 KiB must be enabled per kernel.
 static int allow_large_lds(const svs_plan* pl) {
-  // Autotuner candidates stay within 96 KiB; enable that much once.
-  size_t need = 96 * 1024;
-  for (const LevelPlan& L : pl->levels) need = std::max(need, sweep_lds_bytes(L.bx, L.by));
-  const int v = (int)need;
+  // shape_ok keeps every tile within 96 KiB; enable that much once.
+  (void)pl;
+  const int v = 96 * 1024;
   const void* kernels[] = {
       (const void*)k_sweep<BD_NEAREST, false, false>, (const void*)k_sweep<BD_NEAREST, true, false>,
       (const void*)k_sweep<BD_MIRROR, false, false>,  (const void*)k_sweep<BD_MIRROR, true, false>,
@@ -984,11 +1027,10 @@ int svs_tvl1_plan_create(const svs_tvl1_params* P, int ninst, const int* dims, v
   pl->P.blur_weights = nullptr;  // host copy kept in blur_w
   for (int s = 0; s < pl->S; ++s) {
     LevelPlan& L = pl->levels[s];
-    const size_t lds = sweep_lds_bytes(L.bx, L.by);
-    if (L.bx * L.by > 1024 || L.bx < 3 || L.by < 3 || lds > 160 * 1024 ||
-        2 * (L.bx + 2) + 2 * L.by > L.bx * L.by || L.cz < 1) {
+    if (!shape_ok(L, L.bx, L.by, L.cz)) {
       delete pl;
-      return fail(SVS_STATUS_INVALID, "sweep shape %dx%d not launchable", L.bx, L.by);
+      return fail(SVS_STATUS_INVALID, "sweep shape %dx%d, z-chunk %d not launchable for level %d", L.bx, L.by,
+                  L.cz, s);
     }
   }
   fill_schedule(pl);
@@ -1144,8 +1186,8 @@ int svs_tvl1_plan_autotune(svs_plan* pl, int reps, void* stream, int* report /* 
 
 int svs_tvl1_plan_set_shape(svs_plan* pl, int level, int bx, int by, int cz, void* stream) {
   SVS_REQUIRE(pl && level >= 0 && level < pl->S, "bad plan/level");
-  SVS_REQUIRE(shape_ok(bx, by, cz), "sweep shape %dx%d, z-chunk %d is outside the kernel's limits", bx, by, cz);
   LevelPlan& L = pl->levels[level];
+  SVS_REQUIRE(shape_ok(L, bx, by, cz), "sweep shape %dx%d, z-chunk %d is outside the kernel's limits", bx, by, cz);
   SVS_REQUIRE(count_tiles(L, bx, by, cz) <= L.tile_capacity,
               "sweep shape %dx%d, z-chunk %d needs %d tiles; the plan reserved %d", bx, by, cz,
               count_tiles(L, bx, by, cz), L.tile_capacity);
