@@ -158,7 +158,7 @@ typedef struct svs_tvl1_params {
   int length_fixed;           /* 0: numDofs (L = N - 1); 1: the fixed lengths below */
   float LZ, LY, LX;
   int use_initial_state;      /* num_scales == 1 only: start from the bound u_in, p_in */
-  int sweep_bx, sweep_by, sweep_cz; /* fused-sweep work-group shape; 0 = automatic */
+  int sweep_bx, sweep_by, sweep_cz; /* fused-sweep shape policy (see svs_tvl1_plan_set_shape); 0 = automatic */
 } svs_tvl1_params;
 
 /* Opaque solve plan: pyramid geometry, workspace layout, device tables and an
@@ -207,11 +207,13 @@ int svs_tvl1_plan_sweeps(svs_plan* plan, int level, int count, void* stream);
  * counterpart. */
 int svs_tvl1_plan_autotune(svs_plan* plan, int reps, void* stream, int* report);
 
-/* Sets the fused-sweep shape of one level (work-group bx x by threads, a
- * (bx-2) x (by-2) tile; z-chunk cz) and drops the captured graph: for
- * reproducing a measured configuration.  Fails if the shape exceeds the
- * kernel's limits or the tile table the plan reserved.  Results do not
- * depend on the shape.  No reference counterpart. */
+/* Sets the fused-sweep shape of one level and drops the captured graph: for
+ * reproducing a measured configuration.  The shape is a policy: tiles (plus a
+ * one-voxel ring) at most bx wide within a budget of bx * by threads, marching
+ * cz planes; each instance splits its rows evenly and takes as many rows as
+ * the budget allows, so an instance of width bx - 2 gets exactly bx x by.
+ * Fails if a resulting tile exceeds the kernel's limits or the plan's tile
+ * table.  Results do not depend on the shape.  No reference counterpart. */
 int svs_tvl1_plan_set_shape(svs_plan* plan, int level, int bx, int by, int cz, void* stream);
 
 /* One frame pair without a graph: TVL13DOpticalFlow.compute(I0, I1, u, p). */

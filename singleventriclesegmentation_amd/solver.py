@@ -167,8 +167,10 @@ class Plan:
         return [tuple(rep[3 * i:3 * i + 3]) for i in range(self.cfg.num_scales)]
 
     def set_shapes(self, shapes):
-        """Sets the fused-sweep shape (bx, by, cz) of every level, e.g. to
-        reproduce a configuration the autotuner picked earlier."""
+        """Sets the fused-sweep shape policy (bx, by, cz) of every level, e.g. to
+        reproduce a configuration the autotuner picked earlier: tiles at most
+        bx - 2 voxels wide within bx * by threads, marching cz planes (an
+        instance of width bx - 2 gets bx x by work-groups)."""
         if len(shapes) != self.cfg.num_scales:
             raise RuntimeError(f"need one (bx, by, cz) per level ({self.cfg.num_scales})")
         with torch.cuda.device(self.device):
