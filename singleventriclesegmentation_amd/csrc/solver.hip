@@ -253,10 +253,12 @@ __global__ __launch_bounds__(kBlock) void k_store_output(const LevelDesc* __rest
 // ring and marches along z over [z0, z1).  At plane k it computes p' and
 // r = D p' on the whole extended tile (the ring is recomputed by the
 // neighbouring tile as well), then the primal update of plane k-1 from r of
-// planes k-2..k.  LDS holds u of plane k with a second ring (for the x/y
-// derivatives of the ring) and the x/y derivative columns of r for planes
-// k-1 and k; z-neighbours live in registers of the owning thread.  One
-// barrier per plane.
+// planes k-2..k.  LDS holds, double-buffered, u of plane k with a second ring
+// (for the x/y derivatives of the ring) and the x/y derivative columns of r;
+// a thread's u of plane k-1 is read back from its own slot of the other U
+// buffer, u of planes k and k+1 and the z-column of r live in its registers.
+// One barrier per plane.
+//
 // A work-group's tile: instance, origin, z-range and its own shape, so that
 // one launch can give every instance of an uneven batch full-row tiles.
 struct alignas(16) SweepTile {
@@ -333,10 +335,10 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   const int kstart = z0 > 0 ? z0 - 1 : 0;
   const int kend = z1 < NZ ? z1 : NZ;  // the step k == NZ only finishes plane NZ-1
 
-  // Registers: own-column u of planes k-1..k+1 (k+2 in flight), the dual and
-  // anisotropy of plane k (k+1 in flight), rho_c/g of plane k-1 (k in
-  // flight), the ring value of plane k (k+1 in flight) and the z-derivative
-  // column of r = D p' for planes k-2..k.
+  // Registers: own-column u of planes k-1..k+1 (k+2 in flight; k-1 is
+  // refreshed from LDS each step), the dual and anisotropy of plane k (k+1 in
+  // flight), rho_c/g of plane k-1 (k in flight), the ring value of plane k
+  // (k+1 in flight) and the z-derivative column of r = D p' for planes k-2..k.
   float um[3] = {0.0f, 0.0f, 0.0f}, uc[3] = {0.0f, 0.0f, 0.0f}, up[3] = {0.0f, 0.0f, 0.0f};
   float pc[9], snc[4] = {0.0f, 0.0f, 0.0f, 0.0f}, rgc[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ringc[3] = {0.0f, 0.0f, 0.0f};
   for (int e = 0; e < 9; ++e) pc[e] = 0.0f;
@@ -552,8 +554,13 @@ static void inst_tile(const LevelDesc& d, int bx, int by, int& tbx, int& tby) {
   const int ntx = (d.NX + txmax - 1) / txmax;
   const int tx = (d.NX + ntx - 1) / ntx;  // rows split into equal pieces
   tbx = tx + 2;
-  const int ty = std::min((bx * by) / tbx - 2, d.NY);
-  tby = ty + 2;
+  const int tymax = std::min((bx * by) / tbx - 2, d.NY);
+  if (tymax < 1) {  // not even one row fits the budget: shape_ok rejects it
+    tby = tymax + 2;
+    return;
+  }
+  const int nty = (d.NY + tymax - 1) / tymax;
+  tby = (d.NY + nty - 1) / nty + 2;       // and the rows into equal bands
 }
 
 static int count_tiles(const LevelPlan& L, int bx, int by, int cz) {
@@ -643,24 +650,21 @@ static std::string tune_key(const svs_tvl1_params& P, const LevelPlan& L) {
   return k;
 }
 
-// Sweep launch shape for a level: (BX, BY) threads per work-group (the tile
-// plus its one-voxel ring) and the z-chunk.  Large levels (a batch of many
-// volumes) use full-row tiles 8 rows high marching over long z-chunks; small
-// levels use narrow tiles and short chunks so that enough work-groups exist.
+// Default sweep shape policy of a level, by its voxel count (all instances).
+// Measured on MI355X with the autotuner: batches of large volumes run best
+// with full-row tiles of ~900 threads marching ~40 planes (68 x 80^3:
+// 82 x 11, 40 planes); mid-size levels with rows up to 64 wide and ~600
+// work-groups (68 x 40^3: 42 x 10, 20 planes); a single volume of 10^5..10^6
+// voxels with half rows marching 2 planes (80^3: 42 x 11); small levels are
+// latency-bound and want many small work-groups marching one plane.
 static void choose_sweep_shape(const svs_tvl1_params& P, LevelPlan& L, int ninst) {
   (void)ninst;
-  int maxNX = 0, maxNY = 0, maxNZ = 0;
+  int maxNX = 0;
   int64_t vox = 0;
   for (auto& d : L.host) {
-    maxNX = std::max(maxNX, d.NX); maxNY = std::max(maxNY, d.NY); maxNZ = std::max(maxNZ, d.NZ);
+    maxNX = std::max(maxNX, d.NX);
     vox += d.N;
   }
-  // Measured on MI355X (80^3 levels): a batch of large volumes runs best with
-  // ~40 x 10 interior tiles marching 40 planes; a lone small level with
-  // ~10 x 4 tiles and 2-plane chunks (it is latency-bound, so it wants many
-  // short work-groups).
-  const bool big = vox >= ((int64_t)8 << 20);
-  const bool tiny = vox < ((int64_t)1 << 20);
   int bx = P.sweep_bx, by = P.sweep_by, cz = P.sweep_cz;
   if (bx <= 0 && by <= 0 && cz <= 0) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -670,26 +674,25 @@ static void choose_sweep_shape(const svs_tvl1_params& P, LevelPlan& L, int ninst
       return;
     }
   }
+  int maxtx, budget, minz, target;
+  if (vox >= ((int64_t)8 << 20)) { maxtx = 128; budget = 910; minz = 40; target = 1024; }
+  else if (vox >= ((int64_t)1 << 20)) { maxtx = 64; budget = 448; minz = 4; target = 600; }
+  else if (vox >= ((int64_t)128 << 10)) { maxtx = 40; budget = 462; minz = 2; target = 2048; }
+  else if (vox >= ((int64_t)16 << 10)) { maxtx = 14; budget = 192; minz = 1; target = 2048; }
+  else { maxtx = 8; budget = 64; minz = 1; target = 2048; }
   if (bx <= 0) {
-    const int maxtx = big ? 64 : (tiny ? 10 : 20);
     const int ntx = (maxNX + maxtx - 1) / maxtx;
     bx = (maxNX + ntx - 1) / ntx + 2;
   }
-  if (by <= 0) {
-    int ty = std::min(maxNY, big ? 10 : (tiny ? 4 : 10));
-    while (ty > 1 && bx * (ty + 2) > 1024) --ty;
-    by = ty + 2;
-    // one outer-ring position per thread
-    while (2 * (bx + 2) + 2 * by > bx * by) ++by;
-  }
+  if (by <= 0) by = std::max(3, budget / bx);
   if (!shape_ok(L, bx, by, 1)) {
-    // The policy fits the widest volume; a narrow one may need a taller tile.
+    // A narrow tile may need more rows for one thread per outer-ring position.
     int b = by;
     while (b * bx <= 1024 && !shape_ok(L, bx, b, 1)) ++b;
     if (shape_ok(L, bx, b, 1)) by = b;
     else { bx = 12; by = 12; }
   }
-  if (cz <= 0) cz = default_cz(L, bx, by, big ? 1024 : 2048, big ? 40 : (tiny ? 2 : 4));
+  if (cz <= 0) cz = default_cz(L, bx, by, target, minz);
   L.bx = bx; L.by = by; L.cz = cz;
 }
 
