@@ -396,6 +396,8 @@ def bench_unet(args, dev):
     from singleventriclesegmentation_amd.refinement import Refiner, RefinementConfig, synthetic_batch
     from singleventriclesegmentation_amd.unet import parameter_count
     shape = (args.size,) * 3
+    # RefinementConfig.conv_search (on) lets MIOpen search its convolution
+    # algorithms during the warm-up steps; FP32 throughout.
     batch = synthetic_batch(shape, 4, dev)
     ref = Refiner(shape, 4, dev, RefinementConfig())
     for _ in range(args.warmup):
