@@ -198,7 +198,8 @@ __global__ __launch_bounds__(kBlock) void k_warp_cnn(const float* __restrict__ u
 // Adjoint of one tricubic sample (interpolation_cubicHermiteSpline.cuh:527-636):
 // scatters fwd * d out/d u into grad (atomics, 64 taps) and adds the three
 // flow derivatives to gphi in the reference's order (z, y per dz, x per dy).
-__device__ void warp_backward_cubic(const float* __restrict__ u, float* __restrict__ grad, int NZ, int NY, int NX,
+template <class Acc>
+__device__ void warp_backward_cubic(const float* __restrict__ u, const Acc& acc, int NZ, int NY, int NX,
                                     float hZ, float hY, float hX, int bc, float cz, float cy, float cx, float fwd,
                                     float gphi[3]) {
   const Axis ax = make_axis(cx, hX), ay = make_axis(cy, hY), az = make_axis(cz, hZ);
@@ -233,13 +234,14 @@ __device__ void warp_backward_cubic(const float* __restrict__ u, float* __restri
       const float gwx = cr_local_backward(U(z, y, xs[0]), U(z, y, xs[1]), U(z, y, xs[2]), U(z, y, xs[3]), lx,
                                           gy[dy], gx);
       gphi[0] += gwx / hX;
-      for (int dx = 0; dx < 4; ++dx) unsafeAtomicAdd(grad + ((int64_t)z * NY + y) * NX + xs[dx], gx[dx]);
+      for (int dx = 0; dx < 4; ++dx) acc(z, y, xs[dx], gx[dx]);
     }
   }
 }
 
 // Adjoint of one trilinear sample (interpolation_linear.cuh:297-363).
-__device__ void warp_backward_linear(const float* __restrict__ u, float* __restrict__ grad, int NZ, int NY, int NX,
+template <class Acc>
+__device__ void warp_backward_linear(const float* __restrict__ u, const Acc& acc, int NZ, int NY, int NX,
                                      float hZ, float hY, float hX, int bc, float cz, float cy, float cx, float fv,
                                      float gphi[3]) {
   const Axis ax = make_axis(cx, hX), ay = make_axis(cy, hY), az = make_axis(cz, hZ);
@@ -250,14 +252,14 @@ __device__ void warp_backward_linear(const float* __restrict__ u, float* __restr
   auto I = [&](int z, int y, int x) { return ((int64_t)z * NY + y) * NX + x; };
   const float u_fff = u[I(zf, yf, xf)], u_ffc = u[I(zc, yf, xf)], u_fcf = u[I(zf, yc, xf)], u_fcc = u[I(zc, yc, xf)];
   const float u_cff = u[I(zf, yf, xc)], u_cfc = u[I(zc, yf, xc)], u_ccf = u[I(zf, yc, xc)], u_ccc = u[I(zc, yc, xc)];
-  unsafeAtomicAdd(grad + I(zf, yf, xf), (1 - wz) * (1 - wy) * (1 - wx) * fv);
-  unsafeAtomicAdd(grad + I(zf, yc, xf), (1 - wz) * wy * (1 - wx) * fv);
-  unsafeAtomicAdd(grad + I(zf, yf, xc), (1 - wz) * (1 - wy) * wx * fv);
-  unsafeAtomicAdd(grad + I(zf, yc, xc), (1 - wz) * wy * wx * fv);
-  unsafeAtomicAdd(grad + I(zc, yf, xf), wz * (1 - wy) * (1 - wx) * fv);
-  unsafeAtomicAdd(grad + I(zc, yc, xf), wz * wy * (1 - wx) * fv);
-  unsafeAtomicAdd(grad + I(zc, yf, xc), wz * (1 - wy) * wx * fv);
-  unsafeAtomicAdd(grad + I(zc, yc, xc), wz * wy * wx * fv);
+  acc(zf, yf, xf, (1 - wz) * (1 - wy) * (1 - wx) * fv);
+  acc(zf, yc, xf, (1 - wz) * wy * (1 - wx) * fv);
+  acc(zf, yf, xc, (1 - wz) * (1 - wy) * wx * fv);
+  acc(zf, yc, xc, (1 - wz) * wy * wx * fv);
+  acc(zc, yf, xf, wz * (1 - wy) * (1 - wx) * fv);
+  acc(zc, yc, xf, wz * wy * (1 - wx) * fv);
+  acc(zc, yf, xc, wz * (1 - wy) * wx * fv);
+  acc(zc, yc, xc, wz * wy * wx * fv);
   gphi[0] += ((1 - wz) * (1 - wy) * (u_cff - u_fff) + (1 - wz) * wy * (u_ccf - u_fcf) +
               wz * (1 - wy) * (u_cfc - u_ffc) + wz * wy * (u_ccc - u_fcc)) / hX * fv;
   gphi[1] += ((1 - wz) * (1 - wx) * (u_fcf - u_fff) + (1 - wz) * wx * (u_ccf - u_cff) +
@@ -270,39 +272,103 @@ __device__ void warp_backward_linear(const float* __restrict__ u, float* __restr
 // batched CNN layout: u/grad_u [B,C,Z,Y,X], phi/grad_phi [B,Z,Y,X,3].  The
 // flow gradient sums the per-channel adjoints in channel order.  grad_u must
 // be zero on entry (the launcher clears it).
-__global__ __launch_bounds__(kBlock) void k_warp_backward(const float* __restrict__ u, const float* __restrict__ phi,
-                                const float* __restrict__ gout, int B, int C, int NZ, int NY, int NX, float hZ,
-                                float hY, float hX, int ip, int bc, float* __restrict__ grad_u,
-                                float* __restrict__ grad_phi) {
+//
+// A work-group owns a kAdjTX x kAdjTY x kAdjTZ tile of voxels.  Per channel it
+// sums the scattered grad_u contributions in an LDS window that covers the
+// tile, a displacement margin of kAdjM voxels and the 2-voxel cubic footprint,
+// then adds each non-zero window cell to grad_u with one global atomic.  Taps
+// outside the window (larger displacements, far boundary reflections) go to
+// grad_u directly.  The reference also accumulates with float atomics, so
+// grad_u's low bits depend on the order in either case; grad_phi does not.
+constexpr int kAdjTX = 32, kAdjTY = 8, kAdjTZ = 4, kAdjM = 3;
+constexpr int kAdjThreads = kAdjTX * kAdjTY * kAdjTZ;
+constexpr int kAdjWX = kAdjTX + 2 * (kAdjM + 2), kAdjWY = kAdjTY + 2 * (kAdjM + 2), kAdjWZ = kAdjTZ + 2 * (kAdjM + 2);
+constexpr int kAdjWin = kAdjWX * kAdjWY * kAdjWZ;
+
+struct GlobalAcc {
+  float* grad;
+  int NY, NX;
+  __device__ void operator()(int z, int y, int x, float v) const {
+    unsafeAtomicAdd(grad + ((int64_t)z * NY + y) * NX + x, v);
+  }
+};
+
+struct WindowAcc {
+  float* win;       // LDS, kAdjWZ x kAdjWY x kAdjWX
+  int z0, y0, x0;   // volume index of window cell (0, 0, 0)
+  GlobalAcc outside;
+  __device__ void operator()(int z, int y, int x, float v) const {
+    const int lz = z - z0, ly = y - y0, lx = x - x0;
+    if ((unsigned)lz < (unsigned)kAdjWZ && (unsigned)ly < (unsigned)kAdjWY && (unsigned)lx < (unsigned)kAdjWX)
+      atomicAdd(win + (lz * kAdjWY + ly) * kAdjWX + lx, v);
+    else
+      outside(z, y, x, v);
+  }
+};
+
+__global__ __launch_bounds__(kAdjThreads) void k_warp_backward(const float* __restrict__ u,
+                                                               const float* __restrict__ phi,
+                                                               const float* __restrict__ gout, int B, int C, int NZ,
+                                                               int NY, int NX, float hZ, float hY, float hX, int ip,
+                                                               int bc, float* __restrict__ grad_u,
+                                                               float* __restrict__ grad_phi) {
+  __shared__ float win[kAdjWin];
+  const int tiles_x = (NX + kAdjTX - 1) / kAdjTX, tiles_y = (NY + kAdjTY - 1) / kAdjTY;
+  const int tiles_z = (NZ + kAdjTZ - 1) / kAdjTZ;
+  int tile = blockIdx.x;
+  const int tx = tile % tiles_x;
+  tile /= tiles_x;
+  const int ty = tile % tiles_y;
+  tile /= tiles_y;
+  const int tz = tile % tiles_z;
+  const int b = tile / tiles_z;
+  const int t = threadIdx.x;
+  const int x = tx * kAdjTX + t % kAdjTX, y = ty * kAdjTY + (t / kAdjTX) % kAdjTY, z = tz * kAdjTZ + t / (kAdjTX * kAdjTY);
+  const bool valid = x < NX && y < NY && z < NZ;
   const int64_t N = (int64_t)NZ * NY * NX;
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= (int64_t)B * N) return;
-  const int b = (int)(t / N);
-  const int64_t i = t - (int64_t)b * N;
-  int z, y, x;
-  unflatten(i, NY, NX, z, y, x);
-  const float* ph = phi + ((int64_t)b * N + i) * 3;
-  const float cx = (float)x * hX + ph[0];
-  const float cy = (float)y * hY + ph[1];
-  const float cz = (float)z * hZ + ph[2];
+  const int64_t i = ((int64_t)z * NY + y) * NX + x;
+  float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+  if (valid) {
+    const float* ph = phi + ((int64_t)b * N + i) * 3;
+    cx = (float)x * hX + ph[0];
+    cy = (float)y * hY + ph[1];
+    cz = (float)z * hZ + ph[2];
+  }
+  const int wz0 = tz * kAdjTZ - (kAdjM + 2), wy0 = ty * kAdjTY - (kAdjM + 2), wx0 = tx * kAdjTX - (kAdjM + 2);
   float gphi[3] = {0.0f, 0.0f, 0.0f};
   for (int c = 0; c < C; ++c) {
     const int64_t off = ((int64_t)b * C + c) * N;
-    const float fv = gout[off + i];
-    if (ip == IP_CUBIC) {
-      warp_backward_cubic(u + off, grad_u + off, NZ, NY, NX, hZ, hY, hX, bc, cz, cy, cx, fv, gphi);
-    } else if (ip == IP_LINEAR) {
-      warp_backward_linear(u + off, grad_u + off, NZ, NY, NX, hZ, hY, hX, bc, cz, cy, cx, fv, gphi);
-    } else {
-      const int xx = nearest_axis(cx, hX, NX, bc), yy = nearest_axis(cy, hY, NY, bc),
-                zz = nearest_axis(cz, hZ, NZ, bc);
-      unsafeAtomicAdd(grad_u + off + ((int64_t)zz * NY + yy) * NX + xx, fv);
+    for (int k = t; k < kAdjWin; k += kAdjThreads) win[k] = 0.0f;
+    __syncthreads();
+    if (valid) {
+      const float fv = gout[off + i];
+      const WindowAcc acc{win, wz0, wy0, wx0, GlobalAcc{grad_u + off, NY, NX}};
+      if (ip == IP_CUBIC) {
+        warp_backward_cubic(u + off, acc, NZ, NY, NX, hZ, hY, hX, bc, cz, cy, cx, fv, gphi);
+      } else if (ip == IP_LINEAR) {
+        warp_backward_linear(u + off, acc, NZ, NY, NX, hZ, hY, hX, bc, cz, cy, cx, fv, gphi);
+      } else {
+        acc(nearest_axis(cz, hZ, NZ, bc), nearest_axis(cy, hY, NY, bc), nearest_axis(cx, hX, NX, bc), fv);
+      }
     }
+    __syncthreads();
+    // Only in-volume cells ever receive a contribution (taps are mapped into
+    // the volume first), so a non-zero cell is always a valid voxel.
+    for (int k = t; k < kAdjWin; k += kAdjThreads) {
+      const float v = win[k];
+      if (v != 0.0f) {
+        const int lx = k % kAdjWX, ly = (k / kAdjWX) % kAdjWY, lz = k / (kAdjWX * kAdjWY);
+        unsafeAtomicAdd(grad_u + off + ((int64_t)(wz0 + lz) * NY + (wy0 + ly)) * NX + (wx0 + lx), v);
+      }
+    }
+    __syncthreads();
   }
-  float* gp = grad_phi + ((int64_t)b * N + i) * 3;
-  gp[0] = gphi[0];
-  gp[1] = gphi[1];
-  gp[2] = gphi[2];
+  if (valid) {
+    float* gp = grad_phi + ((int64_t)b * N + i) * 3;
+    gp[0] = gphi[0];
+    gp[1] = gphi[1];
+    gp[2] = gphi[2];
+  }
 }
 
 // --------------------------------------------------------- prolongation
@@ -598,8 +664,11 @@ int svs_warp3d_backward(const float* u, const float* phi, const float* grad_out,
   const int64_t n = (int64_t)NZ * NY * NX;
   int rc = launch_fill(grad_u, (int64_t)B * C * n, 0.0f, (hipStream_t)stream);
   if (rc) return rc;
-  k_warp_backward<<<grid_for((int64_t)B * n, kBlock), kBlock, 0, (hipStream_t)stream>>>(
-      u, phi, grad_out, B, C, NZ, NY, NX, hZ, hY, hX, interp, bdry, grad_u, grad_phi);
+  const int64_t tiles = (int64_t)B * ((NX + kAdjTX - 1) / kAdjTX) * ((NY + kAdjTY - 1) / kAdjTY) *
+                        ((NZ + kAdjTZ - 1) / kAdjTZ);
+  SVS_REQUIRE(tiles < ((int64_t)1 << 31), "Warping3D.backward: too many tiles");
+  k_warp_backward<<<(int)tiles, kAdjThreads, 0, (hipStream_t)stream>>>(u, phi, grad_out, B, C, NZ, NY, NX, hZ, hY,
+                                                                       hX, interp, bdry, grad_u, grad_phi);
   SVS_LAUNCHED();
   return SVS_STATUS_OK;
 }
