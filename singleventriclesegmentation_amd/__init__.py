@@ -9,6 +9,7 @@ in the HIP kernels of libsvsflow.so (built by ``build.build()``).
 from . import opticalFlow  # noqa: F401
 from .flow_utils import (apply_gaussian_blur3d, apply_median_filter, compute_timepoints,  # noqa: F401
                          get_interpolation_type, get_mesh_length)
+from .processor import OpticalFlowProcessor  # noqa: F401
 from .solver import Plan, SolverConfig  # noqa: F401
 from .tvl13d import TVL13DOpticalFlow  # noqa: F401
 from .warping import Warp, propagate  # noqa: F401
