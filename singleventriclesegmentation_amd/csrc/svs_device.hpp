@@ -287,26 +287,13 @@ __device__ __forceinline__ void prox_primal_point(float v[3], float r, const flo
   for (int c = 0; c < 3; ++c) v[c] = v[c] + delta[c];
 }
 
-// Pointers read from a descriptor table are generic (flat) to the compiler;
-// casting them to the global address space lets it address with a wave-
-// uniform SGPR base plus a 32-bit per-lane offset.
-#define SVS_GLOBAL __attribute__((address_space(1)))
-template <class T>
-__device__ __forceinline__ SVS_GLOBAL T* as_global(T* p) {
-  return (SVS_GLOBAL T*)p;
-}
-
-// 16-byte vector load from a global-address-space float4 array.
 typedef float svs_f4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ svs_f4 ld4(const SVS_GLOBAL float4* p, uint32_t i) {
-  return ((const SVS_GLOBAL svs_f4*)p)[i];
-}
 
 // Raw buffer access for the fused sweep.  A 128-bit resource (base address and
 // size in bytes) sits in scalar registers; each access adds a per-lane byte
 // offset (VGPR) and a wave-uniform byte offset (SGPR), so the plane and field
-// offsets cost no vector instructions.  Reads at or past `bytes` return 0 and
-// writes there are dropped.  0x00020000 is the gfx9 word 3 (32-bit data format).
+// offsets cost no vector instructions.  Callers keep every access inside
+// [base, base + bytes).  0x00020000 is the gfx9 word 3 (32-bit data format).
 typedef __amdgpu_buffer_rsrc_t svs_rsrc;
 __device__ __forceinline__ svs_rsrc make_rsrc(const void* base, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
@@ -314,7 +301,6 @@ __device__ __forceinline__ svs_rsrc make_rsrc(const void* base, uint32_t bytes) 
 __device__ __forceinline__ float bld(svs_rsrc r, uint32_t voff, uint32_t soff) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
 }
-typedef unsigned int svs_u4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ svs_f4 bld4(svs_rsrc r, uint32_t voff, uint32_t soff) {
   return __builtin_bit_cast(svs_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
 }
