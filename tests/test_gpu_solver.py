@@ -27,12 +27,12 @@ def make_solver(S=4, W=10, K=50, **kw):
 
 
 def oracle_solve(O, f_I0, f_I1, S, W, K, interpolation="CUBIC_HERMITESPLINE", boundary="MIRROR", aniso=True,
-                 dtype=np.float32):
+                 dtype=np.float32, length_type="numDofs", length=(1, 1, 1)):
     w = gaussian_weights3d(1.0)
     I0 = O.blur(f_I0, w, dtype=dtype)
     I1 = O.blur(f_I1, w, dtype=dtype)
     params = O.solve_params(num_scales=S, max_warps=W, max_outer_iterations=K, interpolation=interpolation,
-                            boundary=boundary, use_aniso=aniso)
+                            boundary=boundary, use_aniso=aniso, length_type=length_type, length=length)
     return O.solve(I0, I1, params, dtype=dtype)
 
 
@@ -60,6 +60,21 @@ def test_fused_matches_oracle_small(O, dev, boundary, aniso):
     assert np.abs(u - uo).max() <= MAX_TOL and e.mean() <= MEAN_TOL
     # Same operation order, no fused multiply-adds, shared expf/powf: identical bits.
     assert np.array_equal(u, uo) and np.array_equal(p, po)
+
+
+@pytest.mark.parametrize("interpolation,length_type,length", [
+    ("NEAREST", "numDofs", (1, 1, 1)), ("LINEAR", "numDofs", (1, 1, 1)),
+    ("CUBIC_HERMITESPLINE", "fixed", (30.0, 22.5, 17.0)), ("LINEAR", "fixed", (9.0, 31.0, 20.0))])
+def test_fused_matches_oracle_interpolation_and_mesh(O, dev, interpolation, length_type, length):
+    """The other interpolation types, and fixed mesh lengths (h != 1 on every axis
+    and level, the general sweep kernel), equal the oracle bit for bit."""
+    shape = (20, 24, 28)
+    I0, I1, _ = pair(shape, seed=3)
+    S, W, K = 2, 3, 10
+    solver = make_solver(S, W, K, interpolation_type=interpolation, length_type=length_type, length=length)
+    u, p = solver.compute(torch.from_numpy(I0).to(dev), torch.from_numpy(I1).to(dev), None, None)
+    uo, po = oracle_solve(O, I0, I1, S, W, K, interpolation=interpolation, length_type=length_type, length=length)
+    assert np.array_equal(u.cpu().numpy(), uo) and np.array_equal(p.cpu().numpy(), po)
 
 
 def test_fused_equals_operator_path(dev):
