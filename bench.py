@@ -42,9 +42,10 @@ def parse():
                          "(reproduces a measured configuration, e.g. under a profiler)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference kernel for PMC byte counters)")
-    ap.add_argument("--workload", default="sequence", choices=["sequence", "uneven", "serve", "unet"],
+    ap.add_argument("--workload", default="sequence", choices=["sequence", "uneven", "serve", "unet", "augment"],
                     help="sequence: 35-frame batch (the headline line); uneven: 96 pairs of mixed sizes; "
-                         "serve: Poisson request service; unet: refinement training with the warp adjoint")
+                         "serve: Poisson request service; unet: refinement training with the warp adjoint; "
+                         "augment: flow-consistent rotate + flips of one 80^3 training sample")
     ap.add_argument("--pairs", type=int, default=96, help="uneven: number of pairs")
     ap.add_argument("--requests", type=int, default=40, help="serve: requests per arrival rate")
     ap.add_argument("--rates", default="5,20,50", help="serve: arrival rates (req/s)")
@@ -415,7 +416,115 @@ def bench_unet(args, dev):
           {"loss": round(float(out["total"].item()), 4)})
 
 
-OTHER = {"uneven": bench_uneven, "serve": bench_serve, "unet": bench_unet}
+AUG_PMC_SUMMARY = os.path.join("profiles", "k_spatial_rotate_35frames_80cubed_pmc.md")
+
+
+def _summary_value(path, key):
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), path)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            k, _, v = line.partition(":")
+            if k.strip() == key:
+                return v.strip()
+    return None
+
+
+def bench_augment(args, dev):
+    """Flow-consistent augmentation of one training sample (TransformsWarper's
+    spatial transforms): RandomRotate + the three RandomFlips on the image
+    sequence, both masks and both flow sequences, fused into one launch."""
+    import numpy as np
+    import torch
+    from singleventriclesegmentation_amd import transforms as T
+
+    shape = (args.size,) * 3
+    F = args.frames
+    g = torch.Generator(device=dev).manual_seed(0)
+    data = {"image": torch.rand((F, 1) + shape, device=dev, generator=g),
+            "med": (torch.rand((1,) + shape, device=dev, generator=g) > 0.5).float(),
+            "mes": (torch.rand((1,) + shape, device=dev, generator=g) > 0.5).float(),
+            "fwd_flow": torch.randn((F - 1, 3) + shape, device=dev, generator=g),
+            "bwd_flow": torch.randn((F - 1, 3) + shape, device=dev, generator=g)}
+    keys = list(data)
+    modes = {"image": "bilinear", "med": "nearest", "mes": "nearest", "fwd_flow": "bilinear",
+             "bwd_flow": "bilinear"}
+    channels = sum(int(np.prod(v.shape[:-3])) for v in data.values())
+    angles = (37.0, 211.0, 118.0)
+    rates = {}
+    for grid in ("device", "host"):
+        rot = T.RandomRotate(keys, 1.0, shape, [(0, 360)] * 3, "zeros", modes, grid=grid, align_corners=False)
+        rot.R = torch.from_numpy(T.rotz(angles[2]) @ T.roty(angles[1]) @ T.rotx(angles[0])).float()
+        step = lambda: T.spatial_pass(dict(data), keys, rotate=rot, flips=7)   # noqa: E731
+        for _ in range(max(1, args.warmup)):
+            step()
+        torch.cuda.synchronize()
+        n = max(args.steps, 10)
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        start.record()
+        for _ in range(n):
+            step()
+        end.record()
+        torch.cuda.synchronize()
+        rates[grid] = (n / (time.perf_counter() - t0), start.elapsed_time(end) / n)
+        log(args, f"augment grid={grid}: {rates[grid][0]:.1f} samples/s, {rates[grid][1]:.3f} ms per sample (events)")
+    if args.copy_mib > 0:
+        src = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=dev).uniform_()
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+    # Algorithmic bytes of one launch: every output channel-voxel written once,
+    # and read once where its sample point lies inside the source volume.
+    R = (T.rotz(angles[2]) @ T.roty(angles[1]) @ T.rotx(angles[0])).astype(np.float64)
+    lin = [np.linspace(-1, 1, s) for s in shape]
+    zz, yy, xx = np.meshgrid(*lin, indexing="ij")
+    pts = np.stack([xx, yy, zz], -1) @ R
+    size = np.array(shape[::-1], np.float64)
+    src = ((pts + 1.0) * size - 1.0) / 2.0
+    inside = int(np.all((src > -1.0) & (src < size), axis=-1).sum())
+    nvox = int(np.prod(shape))
+    alg_bytes = channels * 4 * (nvox + inside)
+    ms = rates["device"][1]
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    # Counters cannot be read inside the timed process: traffic comes from the
+    # committed summary, measured on this workload (80^3, 35 frames).
+    traffic = _summary_value(AUG_PMC_SUMMARY, "traffic_bytes_per_launch") if (shape[0], F) == (80, 35) else None
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        xs = {k: (v[None] if v.ndim == 4 else v).cpu().numpy() for k, v in data.items()}
+        axes = [torch.linspace(-1, 1, s).numpy() for s in shape]
+        Rf = np.ascontiguousarray(rot.R.numpy(), dtype=np.float32)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            for k, x in xs.items():
+                O.spatial_augment(x, Rf, (2, 3, 4), axes, modes[k], "zeros", False, vector=k in T.FLOWS_KEYS)
+            times.append(time.perf_counter() - t0)
+        el = sorted(times)[1]
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": round(1.0 / el, 4), "unit": "samples/s", "cores": cores, "kind": "port",
+               "sample": f"the whole sample ({channels} channels) through the C++ oracle's rotate + 3 flips (the "
+                         f"reference's sequence), median of 3 runs on {cores} OpenMP threads: {el:.2f} s"}
+        log(args, f"cpu oracle: {el:.2f} s per sample")
+    _line(args, "flow-consistent augmentation samples/s (rotate + 3 flips, all keys of a TransformsWarper sample)",
+          round(rates["device"][0], 2), "samples/s", round(rates["device"][1], 4),
+          {"workload": f"{F} frames + 2 masks + 2 x {F - 1} flows of {shape[0]}^3 ({channels} channels), "
+                       f"RandomRotate (bilinear/nearest, zeros, align_corners False) + flips of Z, Y, X in one "
+                       f"launch", "grid": "device", "rotation_deg_xyz": list(angles)},
+          {"host_grid_samples_per_s": round(rates["host"][0], 2),
+           "host_grid_ms_per_sample": round(rates["host"][1], 4),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                        "traffic": int(traffic) if traffic else None,
+                        "traffic_source": f"{AUG_PMC_SUMMARY}: 2 x FETCH_SIZE + WRITE_SIZE" if traffic else None,
+                        "kernel": "k_spatial_rotate", "bytes_per_launch": alg_bytes,
+                        "inside_fraction": round(inside / nvox, 4), "us_per_launch": round(ms * 1e3, 2)},
+           "cpu_baseline": cpu})
+
+
+OTHER = {"uneven": bench_uneven, "serve": bench_serve, "unet": bench_unet, "augment": bench_augment}
 
 
 if __name__ == "__main__":

@@ -135,6 +135,42 @@ size_t svs_mask_hausdorff_workspace_bytes(int NZ, int NY, int NX);
 int svs_mask_hausdorff2(const float* a, const float* b, int NZ, int NY, int NX, void* workspace, size_t ws_bytes,
                         unsigned long long* out, void* stream);
 
+/* ------------------------------------- flow-consistent spatial augmentation */
+
+/* One key of a training sample: src [batch, channels, IZ, IY, IX] ->
+ * dst [batch, channels, OZ, OY, OX], both device, float32, not overlapping.
+ * Images are (T, C, Z, Y, X), masks (C, Z, Y, X) = batch 1, flows
+ * (T, 3, Z, Y, X) with vector = 1 (channels hold the (x, y, z) components). */
+typedef struct svs_aug_volume {
+  const float* src;
+  float* dst;
+  int batch;
+  int channels;
+  int mode;    /* grid_sample mode when rotating: 0 bilinear, 1 nearest */
+  int vector;  /* 1: rotate the 3-vectors by R and negate flipped components */
+} svs_aug_volume;
+
+/* RandomRotate (svs/modules/transforms/spatial.py:53-128) followed by
+ * RandomFlips (spatial.py:142-168), applied to up to 8 keys in one launch.
+ *   R      host 3x3 row-major float32 rotation, or NULL when no rotation was
+ *          drawn (then O* must equal I*);
+ *   grid   device [OZ, OY, OX, 3] = identity_grid @ R as the caller computed
+ *          it (spatial.py:56), or NULL to compute it on the device from
+ *   axes   device lx[OX] ++ ly[OY] ++ lz[OZ], torch.linspace(-1, 1, n) per
+ *          axis (spatial.py:100), multiplied by R here with grid_rounding;
+ *   padding 0 zeros, 1 border, 2 reflection (grid_sample padding_mode);
+ *   flips  bit 0 X (axis 4), bit 1 Y (axis 3), bit 2 Z (axis 2): the XOR of
+ *          the RandomFlips drawn after the rotation;
+ *   grid_rounding, vector_rounding  how the host BLAS behind the reference's
+ *          torch.matmul (grid) and torch.einsum (flow vectors) rounds its
+ *          3-term dot products: 0 fma(a2,b2,fma(a1,b1,fma(a0,b0,+0))) (MKL
+ *          on Intel hosts; the grid from ~110K points up), 1 (a0*b0 + a1*b1)
+ *          + a2*b2 (MKL on AMD hosts), 2 as 1 but summed from +0.
+ * Sampling is torch's CPU grid_sampler_3d, rounding for rounding. */
+int svs_spatial_augment(const svs_aug_volume* volumes, int nvol, int IZ, int IY, int IX, int OZ, int OY, int OX,
+                        const float* R, const float* grid, const float* axes, int padding, int align_corners,
+                        int flips, int grid_rounding, int vector_rounding, void* stream);
+
 /* ---------------------------------------------------- the whole solve */
 
 /* Constructor arguments of TVL13DOpticalFlow (svs/modules/flow/tvl13d.py:14-67)

@@ -253,6 +253,74 @@ def edge_count(m):
     return int(fn(_p(m), *m.shape))
 
 
+PADDING = {"zeros": 0, "border": 1, "reflection": 2}
+SAMPLE_MODE = {"bilinear": 0, "nearest": 1}
+
+
+def rotation_grid(axes, R, rounding=0):
+    """identity_grid @ R (transforms/spatial.py:56, :93-105) as [Z,Y,X,3] (x, y, z),
+    each component a 3-term dot product rounded as the host BLAS does
+    (``rounding``: 0 MKL's FMA chain, 1 plain, 2 plain from +0).
+
+    ``axes`` = (lz, ly, lx), the identity grid's axes: torch.linspace(-1, 1, n)
+    per axis as the reference computes them on the host CPU (spatial.py:100).
+    They are an input here, not restated: torch's vectorised CPU linspace
+    restarts its count at every SIMD chunk, so the values depend on the host's
+    vector width."""
+    lz, ly, lx = (_arr(a, np.float32) for a in axes)
+    OZ, OY, OX = lz.size, ly.size, lx.size
+    R = _arr(R, np.float32)
+    grid = np.empty((OZ, OY, OX, 3), np.float32)
+    lib().orc_rotation_grid(_p(lz), _p(ly), _p(lx), OZ, OY, OX, _p(R), int(rounding), _p(grid))
+    return grid
+
+
+def grid_sample3d(x, grid, mode="bilinear", padding="zeros", align_corners=False):
+    """F.grid_sample(x, grid repeated over the batch) on x [B,C,Z,Y,X] and one
+    grid [OZ,OY,OX,3]."""
+    x = _arr(x, np.float32)
+    grid = _arr(grid, np.float32)
+    B, C, IZ, IY, IX = x.shape
+    OZ, OY, OX = grid.shape[:3]
+    out = np.empty((B, C, OZ, OY, OX), np.float32)
+    lib().orc_grid_sample3d(_p(x), B, C, IZ, IY, IX, _p(grid), OZ, OY, OX, SAMPLE_MODE[mode], PADDING[padding],
+                            int(bool(align_corners)), _p(out))
+    return out
+
+
+def rotate_vectors(v, R, rounding=0):
+    """einsum('ij,tjxyz->tixyz', R, v) (transforms/spatial.py:127) on v [B,3,Z,Y,X],
+    rounded as in rotation_grid."""
+    v = np.array(v, dtype=np.float32, order="C", copy=True)
+    R = _arr(R, np.float32)
+    lib().orc_rotate_vectors(_p(v), v.shape[0], ctypes.c_longlong(int(np.prod(v.shape[2:]))), _p(R),
+                             int(rounding))
+    return v
+
+
+FLIP_COMPONENT = {2: 2, 3: 1, 4: 0}   # transforms/spatial.py:166
+
+
+def spatial_augment(x, R=None, flip_axes=(), axes=None, mode="bilinear", padding="zeros",
+                    align_corners=False, vector=False, grid=None, grid_rounding=0, vector_rounding=0):
+    """RandomRotate with rotation R (None: not drawn) followed by RandomFlip on
+    each axis of ``flip_axes`` in order (transforms/spatial.py:53-90, :142-168),
+    on one key x [B,C,Z,Y,X]; ``vector`` marks a flow key (C == 3); ``axes``
+    as in rotation_grid, or ``grid`` = identity_grid @ R as given; the two
+    roundings as in rotation_grid."""
+    x = _arr(x, np.float32)
+    if R is not None:
+        g = rotation_grid(axes, R, grid_rounding) if grid is None else grid
+        x = grid_sample3d(x, g, mode, padding, align_corners)
+        if vector:
+            x = rotate_vectors(x, R, vector_rounding)
+    for axis in flip_axes:
+        x = np.ascontiguousarray(np.flip(x, axis=axis))
+        if vector:
+            x[:, FLIP_COMPONENT[axis]] *= np.float32(-1.0)
+    return x
+
+
 def solve_params(num_scales=4, max_warps=10, max_outer_iterations=50, weight_matching=1.0, weight_tv=0.025,
                  pd_type=2, pd_params=(0.5, 0.5, 0.25, 1.0), use_aniso=True, aniso_params=(5.0, 0.5),
                  interpolation="CUBIC_HERMITESPLINE", boundary="MIRROR", length_type="numDofs",

@@ -26,6 +26,10 @@
 //   median filter ............ scipy.ndimage.median_filter on CPU
 //   mask metrics ............. scipy.ndimage binary_erosion + distance_transform_edt
 //                              (what monai's Dice/Hausdorff call), closed forms
+//   spatial augmentation ..... torch.nn.functional.grid_sample / torch.matmul /
+//                              torch.einsum on the CPU (bit-exact), closed forms
+//                              (180-degree turns = two flips, identity, rigid
+//                              motion of an affine field)
 //   solve .................... FP64 twin, EPE against a known displacement
 #include <cmath>
 #include <cstdint>
@@ -731,6 +735,122 @@ static long long directed_hausdorff2(const std::vector<int>& ea, const std::vect
   return worst;
 }
 
+// ------------------------------------------------ spatial augmentation
+// transforms/spatial.py.  RandomRotate (:13-128) samples every key with
+// F.grid_sample on the grid G = identity_grid @ R (:56), identity_grid(z,y,x) =
+// (lx[x], ly[y], lz[z]) with l = torch.linspace(-1, 1, n) (:93-105), and
+// rotates flow vectors by R (:115-128).  RandomFlip (:131-168) is torch.flip
+// plus a sign change of one flow component; it is exact and lives in oracle.py.
+//
+// torch.matmul and torch.einsum run on the CPU through the host's BLAS, whose
+// 3-term dot products round in one of three ways (rounding):
+//   0  fma(a2, b2, fma(a1, b1, fma(a0, b0, +0)))   MKL on Intel hosts
+//   1  (a0 * b0 + a1 * b1) + a2 * b2                MKL on AMD hosts
+//   2  ((+0 + a0 * b0) + a1 * b1) + a2 * b2
+// (std::fma is exact; the +0 starts decide the sign of zero results).
+// grid_sample is torch's CPU grid_sampler_3d
+// (aten/src/ATen/native/GridSampler.cpp): scalar float code, restated
+// expression by expression.
+static float dot3(int rounding, float a0, float b0, float a1, float b1, float a2, float b2) {
+  if (rounding == 0) return std::fma(a2, b2, std::fma(a1, b1, std::fma(a0, b0, 0.f)));
+  if (rounding == 1) return (a0 * b0 + a1 * b1) + a2 * b2;
+  return ((0.f + a0 * b0) + a1 * b1) + a2 * b2;
+}
+
+static void rotation_grid(const float* lz, const float* ly, const float* lx, int OZ, int OY, int OX,
+                          const float* R, int rounding, float* grid) {
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < OZ; ++z)
+    for (int y = 0; y < OY; ++y)
+      for (int x = 0; x < OX; ++x) {
+        float* g = grid + (((size_t)z * OY + y) * OX + x) * 3;
+        for (int i = 0; i < 3; ++i) g[i] = dot3(rounding, lx[x], R[i], ly[y], R[3 + i], lz[z], R[6 + i]);
+      }
+}
+
+// v[b, i] <- sum_j R[i][j] v[b, j] on [B, 3, N] (einsum 'ij,tjxyz->tixyz').
+static void rotate_vectors(float* v, int B, size_t N, const float* R, int rounding) {
+#pragma omp parallel for schedule(static)
+  for (int b = 0; b < B; ++b)
+    for (size_t n = 0; n < N; ++n) {
+      float* p = v + (size_t)b * 3 * N + n;
+      const float a0 = p[0], a1 = p[N], a2 = p[2 * N];
+      for (int i = 0; i < 3; ++i) p[(size_t)i * N] = dot3(rounding, R[3 * i], a0, R[3 * i + 1], a1, R[3 * i + 2], a2);
+    }
+}
+
+// grid_sampler_unnormalize / clip_coordinates / reflect_coordinates.
+static float gs_unnormalize(float c, int size, bool align) {
+  return align ? ((c + 1.f) / 2.f) * (float)(size - 1) : ((c + 1.f) * (float)size - 1.f) / 2.f;
+}
+static float gs_clip(float c, int size) { return std::min((float)(size - 1), std::max(c, 0.f)); }
+// safe_downgrade_to_int_range: non-finite or out-of-int-range -> -100 (outside).
+static float gs_int_range(float c) {
+  return (c > (float)(INT32_MAX - 1) || c < (float)INT32_MIN || !std::isfinite(c)) ? -100.f : c;
+}
+static float gs_reflect(float in, int twice_low, int twice_high) {
+  if (twice_low == twice_high) return 0.f;
+  const float lo = (float)twice_low / 2.f, span = (float)(twice_high - twice_low) / 2.f;
+  in = std::fabs(in - lo);
+  const float extra = std::fmod(in, span);
+  const int flips = (int)std::floor(in / span);
+  return (flips % 2 == 0) ? extra + lo : span - extra + lo;
+}
+// padding: 0 zeros, 1 border, 2 reflection.
+static float gs_source(float c, int size, int padding, bool align) {
+  c = gs_unnormalize(c, size, align);
+  if (padding == 1) {
+    c = gs_clip(c, size);
+  } else if (padding == 2) {
+    c = align ? gs_reflect(c, 0, 2 * (size - 1)) : gs_reflect(c, -1, 2 * size - 1);
+    c = gs_clip(c, size);
+  }
+  return gs_int_range(c);
+}
+
+// x [B, C, IZ, IY, IX] sampled at grid [OZ, OY, OX, 3] (shared by the batch,
+// as the repeated grid of spatial.py:79) -> out [B, C, OZ, OY, OX].
+// mode: 0 bilinear (trilinear in 3D), 1 nearest.
+static void grid_sample3d(const float* x, int B, int C, int IZ, int IY, int IX, const float* grid, int OZ,
+                          int OY, int OX, int mode, int padding, bool align, float* out) {
+  const size_t Nin = (size_t)IZ * IY * IX, Nout = (size_t)OZ * OY * OX;
+  auto inside = [&](long z, long y, long x) { return z >= 0 && z < IZ && y >= 0 && y < IY && x >= 0 && x < IX; };
+#pragma omp parallel for schedule(static) collapse(2)
+  for (int bc = 0; bc < B * C; ++bc)
+    for (int z = 0; z < OZ; ++z) {
+      const float* in = x + (size_t)bc * Nin;
+      for (int y = 0; y < OY; ++y)
+        for (int xo = 0; xo < OX; ++xo) {
+          const size_t o = ((size_t)z * OY + y) * OX + xo;
+          const float ix = gs_source(grid[3 * o], IX, padding, align);
+          const float iy = gs_source(grid[3 * o + 1], IY, padding, align);
+          const float iz = gs_source(grid[3 * o + 2], IZ, padding, align);
+          auto at = [&](long zz, long yy, long xx) { return in[((size_t)zz * IY + yy) * IX + xx]; };
+          float r = 0.f;
+          if (mode == 1) {
+            const long xn = (long)std::nearbyint(ix), yn = (long)std::nearbyint(iy), zn = (long)std::nearbyint(iz);
+            r = inside(zn, yn, xn) ? at(zn, yn, xn) : 0.f;
+          } else {
+            const long x0 = (long)std::floor(ix), y0 = (long)std::floor(iy), z0 = (long)std::floor(iz);
+            const long x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
+            const float wx0 = (float)x1 - ix, wx1 = ix - (float)x0;
+            const float wy0 = (float)y1 - iy, wy1 = iy - (float)y0;
+            const float wz0 = (float)z1 - iz, wz1 = iz - (float)z0;
+            // (t|b)(n|s)(w|e): z0|z1, y0|y1, x0|x1; weights multiplied x, y, z.
+            if (inside(z0, y0, x0)) r += at(z0, y0, x0) * (wx0 * wy0 * wz0);
+            if (inside(z0, y0, x1)) r += at(z0, y0, x1) * (wx1 * wy0 * wz0);
+            if (inside(z0, y1, x0)) r += at(z0, y1, x0) * (wx0 * wy1 * wz0);
+            if (inside(z0, y1, x1)) r += at(z0, y1, x1) * (wx1 * wy1 * wz0);
+            if (inside(z1, y0, x0)) r += at(z1, y0, x0) * (wx0 * wy0 * wz1);
+            if (inside(z1, y0, x1)) r += at(z1, y0, x1) * (wx1 * wy0 * wz1);
+            if (inside(z1, y1, x0)) r += at(z1, y1, x0) * (wx0 * wy1 * wz1);
+            if (inside(z1, y1, x1)) r += at(z1, y1, x1) * (wx1 * wy1 * wz1);
+          }
+          out[(size_t)bc * Nout + o] = r;
+        }
+    }
+}
+
 // TVL13DOpticalFlow.compute, tvl13d.py:69-162.  I0/I1 are already blurred
 // when the caller wants the blur (the blur weights come from flow_utils.py).
 template <class T> static void solve(const SolveParams& P, const T* I0, const T* I1, int NZ, int NY, int NX,
@@ -858,6 +978,18 @@ extern "C" void orc_mask_hausdorff2(const float* a, const float* b, int NZ, int 
 }
 extern "C" long long orc_mask_edge_count(const float* m, int NZ, int NY, int NX) {
   return (long long)(mask_edges(m, NZ, NY, NX).size() / 3);
+}
+
+extern "C" void orc_rotation_grid(const float* lz, const float* ly, const float* lx, int OZ, int OY, int OX,
+                                  const float* R, int rounding, float* grid) {
+  rotation_grid(lz, ly, lx, OZ, OY, OX, R, rounding, grid);
+}
+extern "C" void orc_rotate_vectors(float* v, int B, long long N, const float* R, int rounding) {
+  rotate_vectors(v, B, (size_t)N, R, rounding);
+}
+extern "C" void orc_grid_sample3d(const float* x, int B, int C, int IZ, int IY, int IX, const float* grid, int OZ,
+                                  int OY, int OX, int mode, int padding, int align, float* out) {
+  grid_sample3d(x, B, C, IZ, IY, IX, grid, OZ, OY, OX, mode, padding, align != 0, out);
 }
 
 extern "C" void orc_step_schedule(const double* params, double* sig, double* tau) {

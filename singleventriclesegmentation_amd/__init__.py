@@ -6,7 +6,7 @@ module, ``TVL13DOpticalFlow`` the solver driver, ``Warp``/``propagate`` the
 mask warping harness and ``warp`` the differentiable warp.  All compute runs
 in the HIP kernels of libsvsflow.so (built by ``build.build()``).
 """
-from . import opticalFlow  # noqa: F401
+from . import opticalFlow, transforms  # noqa: F401
 from .flow_utils import (apply_gaussian_blur3d, apply_median_filter, compute_timepoints,  # noqa: F401
                          get_interpolation_type, get_mesh_length)
 from .processor import OpticalFlowProcessor  # noqa: F401

@@ -37,6 +37,12 @@ class TVL1Params(ctypes.Structure):
     ]
 
 
+class AugVolume(ctypes.Structure):
+    """Mirror of svs_aug_volume (include/svsflow.h)."""
+
+    _fields_ = [("src", _vp), ("dst", _vp), ("batch", _i), ("channels", _i), ("mode", _i), ("vector", _i)]
+
+
 _SIGNATURES = {
     "svs_last_error": ([], ctypes.c_char_p),
     "svs_fill": ([_vp, _i64, _f, _vp], _i),
@@ -55,6 +61,8 @@ _SIGNATURES = {
     "svs_mask_counts": ([_vp, _vp, _i, _i, _i, _vp, _vp], _i),
     "svs_mask_hausdorff_workspace_bytes": ([_i, _i, _i], ctypes.c_size_t),
     "svs_mask_hausdorff2": ([_vp, _vp, _i, _i, _i, _vp, ctypes.c_size_t, _vp, _vp], _i),
+    "svs_spatial_augment": ([ctypes.POINTER(AugVolume), _i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _i, _i,
+                             _i, _i, _vp], _i),
     "svs_tvl1_workspace_bytes": ([ctypes.POINTER(TVL1Params), _i, ctypes.POINTER(_i)], ctypes.c_size_t),
     "svs_tvl1_plan_create": ([ctypes.POINTER(TVL1Params), _i, ctypes.POINTER(_i), _vp, ctypes.c_size_t, _vp,
                               ctypes.POINTER(_vp)], _i),
