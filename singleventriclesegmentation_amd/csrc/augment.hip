@@ -23,19 +23,19 @@
 //     out-of-volume corners skipped.
 //
 // Rotation kernel: a work-group owns an 8 x 8 x 8 tile of rotated-volume
-// positions q and a run of channels.  It computes the source bounding box of
-// the tile once (a rigid motion of an 8^3 cube spans at most 15^3 source
-// voxels), stages that box of each channel in LDS with coalesced loads
-// (double-buffered: the next channel's box is in flight while the current
-// one is sampled), with zeros where the box leaves the volume, and gathers
-// the 8 trilinear corners from LDS without bounds tests.  Staging loads and
-// output stores are buffer instructions on per-channel descriptors with
-// offsets computed once per tile: an offset past the buffer (apron
-// elements, voxels outside the output) loads 0 or drops the store.  A tile whose box
-// does not fit (resampling to a coarser grid, reflection folds) gathers
-// straight from global memory instead.  Work-groups are handed to the 8 XCDs
-// round-robin, so wg = (block % 8) * per_xcd + block / 8 gives each XCD a
-// contiguous run of neighbouring tiles whose boxes overlap in its L2.
+// positions q and streams about 8 channels.  It computes the source
+// coordinates, weights and bounding box of the tile once (a rigid motion of
+// an 8^3 cube spans at most 14^3 source voxels), stages that box of each
+// channel in LDS with coalesced loads (two channels ahead in flight while
+// the current one is sampled), with zeros where the box leaves the volume,
+// and gathers the 8 trilinear corners from LDS without bounds tests.
+// Staging loads and output stores are buffer instructions on per-channel
+// descriptors with offsets computed once per tile: an offset past the buffer
+// (apron elements, voxels outside the output) loads 0 or drops the store.  A
+// tile whose box does not fit (resampling to a coarser grid, reflection
+// folds) gathers straight from global memory instead.  Work-groups run in plain block
+// order (tile fastest, then channel group): giving each XCD a contiguous run
+// of tiles measured 6 % slower.
 #include <algorithm>
 #include <climits>
 
@@ -46,17 +46,14 @@
 namespace svs {
 
 constexpr int kAugMaxVolumes = 8;
-constexpr int kAugXcds = 8;
 constexpr int kFlipTX = 16, kFlipTY = 16;
 constexpr int kRotEdge = 8;
 constexpr int kRotThreads = 256;
 constexpr int kRotVox = kRotEdge * kRotEdge * kRotEdge / kRotThreads;   // voxels per thread
-constexpr int kStage = 14;                                               // staged floats per thread
-constexpr int kBoxMax = kStage * kRotThreads;                            // 3584 >= 15^3
-#ifndef SVS_AUG_CHANNELS_PER_GROUP
-#define SVS_AUG_CHANNELS_PER_GROUP 6
-#endif
-constexpr int kRotChannelsPerGroup = SVS_AUG_CHANNELS_PER_GROUP;          // channels a work-group streams
+constexpr int kStage = 12;                                               // staged floats per thread
+constexpr int kBoxMax = kStage * kRotThreads;                            // 3072 >= 14^3
+constexpr int kLdsStride = 3;                                            // dwords per staged element
+constexpr int kRotChannelsPerGroup = 8;                                  // channels a work-group streams
 
 struct AugVolume {
   const float* src;
@@ -73,7 +70,7 @@ struct AugArgs {
   float R[9];
   const float* grid;   // [OZ, OY, OX, 3] or null
   const float* axes;   // lx[OX], ly[OY], lz[OZ] when grid is null
-  int tiles_x, tiles_y, ntiles, units_per_group, total, per_xcd;
+  int tiles_x, tiles_y, ntiles, units_per_group, total;
 };
 
 // grid_sampler_unnormalize / clip_coordinates / reflect_coordinates /
@@ -124,9 +121,7 @@ __device__ __forceinline__ int volume_of(const AugArgs& a, int unit) {
 // One thread per voxel of one (key, batch entry) item; a tile is 16 x 16
 // voxels of one z plane.
 __global__ __launch_bounds__(kFlipTX* kFlipTY) void k_spatial_flip(const AugArgs a) {
-  const int k = (int)(blockIdx.x / kAugXcds);
-  const int tile = (int)(blockIdx.x % kAugXcds) * a.per_xcd + k;
-  if (tile >= a.total) return;
+  const int tile = (int)blockIdx.x;
   const int item = tile / a.ntiles;
   int r = tile - item * a.ntiles;
   const int plane = a.tiles_x * a.tiles_y;
@@ -200,13 +195,17 @@ __device__ __forceinline__ void store_vector(const AugArgs& a, svs_rsrc dst, uin
   bst(dst, off, 2 * comp_bytes, r2);
 }
 
-// ---- staged path.  LDS holds the two staging buffers interleaved: element
-// e of buffer b is dword 2e + b, so a corner pair (e, e + 1) of buffer b is
-// one ds_read2_b32 at byte 8e with dword offsets b and b + 2.
+// ---- staged path.  LDS holds two staging buffers interleaved with a
+// stride of 3 dwords: element e of buffer b is dword 3e + b, so a corner pair
+// (e, e + 1) of buffer b is one ds_read2_b32 at byte 12e with dword offsets b
+// and b + 3 (a stride coprime with the bank count, so random pairs spread over
+// all banks).  Two channels are in flight in registers: while channel k is
+// sampled from LDS, channel k + 1 (loaded a step earlier) waits in registers
+// and channel k + 2 is being loaded.
 struct VoxL {
   float w[8];        // trilinear corner weights, corner k = (dz << 2) | (dy << 1) | dx
-  uint32_t pair[4];  // byte address 8 e of the corner pairs q = (dz << 1) | dy (dx = 0, 1)
-  uint32_t near;     // byte address 8 e of the nearest neighbour
+  uint32_t pair[4];  // byte address 12 e of the corner pairs q = (dz << 1) | dy (dx = 0, 1)
+  uint32_t near;     // byte address 12 e of the nearest neighbour
   uint32_t out;      // byte offset of flip(q) in an output channel, kOutside when q is not inside
 };
 
@@ -218,18 +217,37 @@ __device__ __forceinline__ float sample_box(const float* stage, const VoxL& x, i
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const float* p = reinterpret_cast<const float*>(sb + x.pair[q]);
-    const float v0 = p[B], v1 = p[B + 2];
+    const float v0 = p[B], v1 = p[B + kLdsStride];
     acc = acc + v0 * x.w[2 * q];
     acc = acc + v1 * x.w[2 * q + 1];
   }
   return acc;
 }
 
-struct StagedLoop {
+// Position (unit, component) in the work-group's channel sequence.
+struct ChanPos {
   int u, comp;
-  Chan cur;
+  bool valid;
+  Chan c;
+};
+
+__device__ __forceinline__ ChanPos chan_next(const AugArgs& a, const ChanPos& p, int u1, size_t Nin, size_t Nout) {
+  ChanPos n;
+  n.u = p.u;
+  n.comp = p.comp + 1;
+  if (n.comp >= p.c.ncomp) {
+    n.u = p.u + 1;
+    n.comp = 0;
+  }
+  n.valid = p.valid && n.u < u1;
+  n.c = n.valid ? chan_at(a, n.u, n.comp, Nin, Nout) : p.c;
+  return n;
+}
+
+struct StagedLoop {
+  ChanPos cur, nxt, nxt2;   // channels k, k + 1, k + 2
   float vec[kRotVox][3];
-  float sv[kStage];
+  float sv[2][kStage];      // register stages: [k & 1] = channel k + 2, [(k + 1) & 1] = channel k + 1
 };
 
 __device__ __forceinline__ void stage_load(const Chan& c, size_t Nin, const uint32_t (&goff)[kStage],
@@ -241,50 +259,43 @@ __device__ __forceinline__ void stage_load(const Chan& c, size_t Nin, const uint
 
 template <int B>
 __device__ __forceinline__ void stage_store(float* stage, const float (&sv)[kStage]) {
-  char* sb = reinterpret_cast<char*>(stage) + 8 * threadIdx.x + 4 * B;
+  char* sb = reinterpret_cast<char*>(stage) + 4 * kLdsStride * threadIdx.x + 4 * B;
 #pragma unroll
-  for (int s = 0; s < kStage; ++s) *reinterpret_cast<float*>(sb + 8 * kRotThreads * s) = sv[s];
+  for (int s = 0; s < kStage; ++s) *reinterpret_cast<float*>(sb + 4 * kLdsStride * kRotThreads * s) = sv[s];
 }
 
-// Samples the channel staged in buffer B, prefetches the next channel and
-// stages it in buffer B ^ 1.  Returns false after the last channel.
+// Samples channel k (staged in LDS buffer B), starts the loads of channel
+// k + 2, stages channel k + 1 in buffer B ^ 1.  Returns false after the last
+// channel.
 template <int B>
 __device__ __forceinline__ bool staged_step(const AugArgs& a, StagedLoop& st, const VoxL (&vx)[kRotVox],
                                             const uint32_t (&goff)[kStage], int u1, float* stage, size_t Nin,
                                             size_t Nout) {
-  int nu = st.u, ncomp = st.comp + 1;
-  if (ncomp >= st.cur.ncomp) {
-    nu = st.u + 1;
-    ncomp = 0;
-  }
-  const bool more = nu < u1;
-  Chan nxt = st.cur;
-  if (more) {
-    nxt = chan_at(a, nu, ncomp, Nin, Nout);
-    stage_load(nxt, Nin, goff, st.sv);
-  }
+  if (st.nxt2.valid) stage_load(st.nxt2.c, Nin, goff, st.sv[B]);
+  const Chan& c = st.cur.c;
+  const int comp = st.cur.comp;
   const uint32_t comp_bytes = (uint32_t)(Nout * 4);
-  const svs_rsrc dst = make_rsrc(st.cur.dst, (st.cur.vector ? 3u : 1u) * comp_bytes);
+  const svs_rsrc dst = make_rsrc(c.dst, (c.vector ? 3u : 1u) * comp_bytes);
 #pragma unroll
   for (int j = 0; j < kRotVox; ++j) {
-    const float val = sample_box<B>(stage, vx[j], st.cur.mode);
+    const float val = sample_box<B>(stage, vx[j], c.mode);
     // Every component is rewritten on every path (a select, no branch).
-    st.vec[j][0] = st.comp == 0 ? val : st.vec[j][0];
-    st.vec[j][1] = st.comp == 1 ? val : st.vec[j][1];
-    st.vec[j][2] = st.comp == 2 ? val : st.vec[j][2];
-    if (!st.cur.vector) bst(dst, vx[j].out, 0, val);
+    st.vec[j][0] = comp == 0 ? val : st.vec[j][0];
+    st.vec[j][1] = comp == 1 ? val : st.vec[j][1];
+    st.vec[j][2] = comp == 2 ? val : st.vec[j][2];
+    if (!c.vector) bst(dst, vx[j].out, 0, val);
   }
-  if (st.cur.vector && st.comp == 2) {
+  if (c.vector && comp == 2) {
 #pragma unroll
     for (int j = 0; j < kRotVox; ++j)
       store_vector(a, dst, vx[j].out, comp_bytes, st.vec[j][0], st.vec[j][1], st.vec[j][2]);
   }
-  if (!more) return false;
-  stage_store<B ^ 1>(stage, st.sv);
+  if (!st.nxt.valid) return false;
+  stage_store<B ^ 1>(stage, st.sv[B ^ 1]);
   __syncthreads();
-  st.u = nu;
-  st.comp = ncomp;
-  st.cur = nxt;
+  st.cur = st.nxt;
+  st.nxt = st.nxt2;
+  st.nxt2 = chan_next(a, st.nxt2, u1, Nin, Nout);
   return true;
 }
 
@@ -343,10 +354,9 @@ __device__ void global_channels(const AugArgs& a, const VoxG (&vx)[kRotVox], int
 }
 
 __global__ __launch_bounds__(kRotThreads) void k_spatial_rotate(const AugArgs a) {
-  __shared__ float stage[2 * kBoxMax];
+  __shared__ float stage[kLdsStride * kBoxMax];
   __shared__ int part[kRotThreads / 64][6];
-  const int k = (int)(blockIdx.x / kAugXcds);
-  const int wg = (int)(blockIdx.x % kAugXcds) * a.per_xcd + k;
+  const int wg = (int)blockIdx.x;
   if (wg >= a.total) return;   // uniform per work-group, before any barrier
   const int group = wg / a.ntiles, tile = wg - group * a.ntiles;
   const int u0 = group * a.units_per_group;
@@ -454,8 +464,8 @@ __global__ __launch_bounds__(kRotThreads) void k_spatial_rotate(const AugArgs a)
       // are at least 2 long, so they stay in the box).
       const int base = in[j] ? z0 * dz + y0 * dy + x0 : 0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) x.pair[q] = 8u * (uint32_t)(base + (q & 1) * dy + (q >> 1) * dz);
-      x.near = in[j] ? 8u * (uint32_t)(zn * dz + yn * dy + xn) : 0u;
+      for (int q = 0; q < 4; ++q) x.pair[q] = 4u * kLdsStride * (uint32_t)(base + (q & 1) * dy + (q >> 1) * dz);
+      x.near = in[j] ? 4u * kLdsStride * (uint32_t)(zn * dz + yn * dy + xn) : 0u;
       x.out = out[j];
     }
     // Source byte offsets of this thread's share of the box (kOutside where
@@ -491,13 +501,17 @@ __global__ __launch_bounds__(kRotThreads) void k_spatial_rotate(const AugArgs a)
       }
     }
     StagedLoop st;
-    st.u = u0;
-    st.comp = 0;
-    st.cur = chan_at(a, u0, 0, Nin, Nout);
+    st.cur.u = u0;
+    st.cur.comp = 0;
+    st.cur.valid = true;
+    st.cur.c = chan_at(a, u0, 0, Nin, Nout);
+    st.nxt = chan_next(a, st.cur, u1, Nin, Nout);
+    st.nxt2 = chan_next(a, st.nxt, u1, Nin, Nout);
 #pragma unroll
     for (int j = 0; j < kRotVox; ++j) st.vec[j][0] = st.vec[j][1] = st.vec[j][2] = 0.f;
-    stage_load(st.cur, Nin, goff, st.sv);
-    stage_store<0>(stage, st.sv);
+    stage_load(st.cur.c, Nin, goff, st.sv[0]);
+    if (st.nxt.valid) stage_load(st.nxt.c, Nin, goff, st.sv[1]);
+    stage_store<0>(stage, st.sv[0]);
     __syncthreads();
     while (staged_step<0>(a, st, vx, goff, u1, stage, Nin, Nout) &&
            staged_step<1>(a, st, vx, goff, u1, stage, Nin, Nout)) {
@@ -620,15 +634,15 @@ extern "C" int svs_spatial_augment(const svs_aug_volume* volumes, int nvol, int 
     SVS_REQUIRE(total < ((int64_t)1 << 30), "spatial augment: %lld tiles exceed the launch limit", (long long)total);
     a.ntiles = (int)per_item;
     a.total = (int)total;
-    a.per_xcd = (int)((total + kAugXcds - 1) / kAugXcds);
-    k_spatial_flip<<<(unsigned)a.per_xcd * kAugXcds, kFlipTX * kFlipTY, 0, st>>>(a);
+    k_spatial_flip<<<(unsigned)total, kFlipTX * kFlipTY, 0, st>>>(a);
   } else {
     a.tiles_x = (OX + kRotEdge - 1) / kRotEdge;
     a.tiles_y = (OY + kRotEdge - 1) / kRotEdge;
     const int64_t ntiles = (int64_t)a.tiles_x * a.tiles_y * ((OZ + kRotEdge - 1) / kRotEdge);
-    // A few channels per work-group keep the work-groups resident on an XCD
-    // (neighbouring tiles) on the same channels, so their overlapping boxes
-    // are read from HBM about once.
+    // About 8 channels per work-group: enough to spread the per-tile set-up,
+    // few enough that work-groups on neighbouring tiles stay within a few
+    // channels of each other (measured on 80^3: 4, 8, 16, 32 and all 241
+    // channels per work-group take 0.90, 0.85, 0.87, 0.98 and 1.01 ms).
     const int64_t upg = std::max<int64_t>(1, (units * kRotChannelsPerGroup + channels / 2) / channels);
     const int64_t groups = (units + upg - 1) / upg;
     const int64_t total = ntiles * groups;
@@ -637,8 +651,7 @@ extern "C" int svs_spatial_augment(const svs_aug_volume* volumes, int nvol, int 
     a.ntiles = (int)ntiles;
     a.units_per_group = (int)upg;
     a.total = (int)total;
-    a.per_xcd = (int)((total + kAugXcds - 1) / kAugXcds);
-    k_spatial_rotate<<<(unsigned)a.per_xcd * kAugXcds, kRotThreads, 0, st>>>(a);
+    k_spatial_rotate<<<(unsigned)total, kRotThreads, 0, st>>>(a);
   }
   SVS_LAUNCHED();
   return SVS_STATUS_OK;
