@@ -42,10 +42,12 @@ def parse():
                          "(reproduces a measured configuration, e.g. under a profiler)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference kernel for PMC byte counters)")
-    ap.add_argument("--workload", default="sequence", choices=["sequence", "uneven", "serve", "unet", "augment"],
+    ap.add_argument("--workload", default="sequence",
+                    choices=["sequence", "uneven", "serve", "unet", "augment", "train"],
                     help="sequence: 35-frame batch (the headline line); uneven: 96 pairs of mixed sizes; "
                          "serve: Poisson request service; unet: refinement training with the warp adjoint; "
-                         "augment: flow-consistent rotate + flips of one 80^3 training sample")
+                         "augment: flow-consistent rotate + flips of one 80^3 training sample; "
+                         "train: mask-propagation training steps (lit_unet.py) on GPU-augmented batches")
     ap.add_argument("--pairs", type=int, default=96, help="uneven: number of pairs")
     ap.add_argument("--requests", type=int, default=40, help="serve: requests per arrival rate")
     ap.add_argument("--rates", default="5,20,50", help="serve: arrival rates (req/s)")
@@ -524,7 +526,73 @@ def bench_augment(args, dev):
            "cpu_baseline": cpu})
 
 
-OTHER = {"uneven": bench_uneven, "serve": bench_serve, "unet": bench_unet, "augment": bench_augment}
+def bench_train(args, dev):
+    """Mask-propagation training (svs/models/lit_unet.py with conf/model/unet.yaml):
+    batch 4 of 80^3 patients with ED->ES chains of 8-11 flows (flows from the
+    batched TV-L1 solver), every sample augmented on the GPU (rotation + 3
+    flips) and collated, then both propagation chains, the loss, backward and
+    the Adam step."""
+    import numpy as np
+    import torch
+    from singleventriclesegmentation_amd import training as TR
+    from singleventriclesegmentation_amd import transforms as T
+    from singleventriclesegmentation_amd.solver import SolverConfig
+
+    shape = (args.size,) * 3
+    t0 = time.perf_counter()
+    data = TR.SyntheticPropagationData(shape, frames=12, n=4, device=dev, gaps=(8, 9, 10, 11),
+                                       solver_cfg=SolverConfig(num_scales=4, max_warps=10, max_outer_iterations=50))
+    torch.cuda.synchronize()
+    log(args, f"train: 76 flows of {shape} solved in {time.perf_counter() - t0:.2f} s")
+    keys = ["image", "mi", "mf", "fwd_flow", "bwd_flow"]
+    modes = {"image": "bilinear", "mi": "nearest", "mf": "nearest", "fwd_flow": "bilinear", "bwd_flow": "bilinear"}
+    aug = T.Compose([T.RandomRotate(keys, 1.0, shape, [(0, 360)] * 3, "zeros", modes, grid="device",
+                                    align_corners=False)] + [T.RandomFlip(keys, 0.5, a) for a in (2, 3, 4)])
+    module = TR.PropagationModule(epochs=10, lambda_u=1.0, gamma_p=0.1, lr=1e-3, betas=(0.9, 0.999), weight_decay=1e-5,
+                                  model_kwargs=dict(num_layers=4, num_classes=1, input_channels=2, features_start=8,
+                                                    trilinear=False, padding=1, kernel_size=[3, 3, 3]),
+                                  warp_kwargs=dict(interpolation_type="LINEAR", boundary_type="MIRROR",
+                                                   mesh_length_type="numDofs", lenghts=(1, 1, 1))).to(dev)
+    opt, _ = module.configure_optimizers()
+    np.random.seed(0)
+
+    def make_batch():
+        return TR.collate_fn([data.item(k, aug) for k in range(len(data))])
+
+    def train_step(batch):
+        opt.zero_grad(set_to_none=True)
+        loss, _ = module.forward_and_loss(batch)
+        loss["t"].backward()
+        opt.step()
+        return loss
+
+    for _ in range(max(1, args.warmup)):
+        train_step(make_batch())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss = train_step(make_batch())
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    batch = make_batch()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        train_step(batch)
+    torch.cuda.synchronize()
+    el_train = time.perf_counter() - t1
+    ts = int(batch["fwd_flow"].shape[1])
+    _line(args, "mask-propagation training steps/s (batch 4 of 80^3, GPU augmentation + both chains + Adam)",
+          round(args.steps / el, 3), "steps/s", round(1e3 * el / args.steps, 2),
+          {"workload": f"batch 4 x {shape[0]}^3, chains of {ts} flows (ED->ES 8..11, padded), U-Net 4 levels / "
+                       f"8 channels, LINEAR warp, rotation + 3 flips per sample on the GPU",
+           "unet_calls_per_step": 2 * ts},
+          {"train_only_steps_per_s": round(args.steps / el_train, 3),
+           "loss": round(float(loss["t"].detach()), 4), "samples_per_s": round(4 * args.steps / el, 2)})
+
+
+OTHER = {"uneven": bench_uneven, "serve": bench_serve, "unet": bench_unet, "augment": bench_augment,
+         "train": bench_train}
 
 
 if __name__ == "__main__":

@@ -12,7 +12,8 @@ flow adjoint part of every step.
 
 The reference trains with a LINEAR warp that has no backward, so gradients
 stop at the warp (svs/models/lit_unet.py:114, warpingOpsCNN.h:36-41); here
-they flow through it.
+they flow through it.  This objective extends the reference; its own training
+loop (LitUNet, PropagationLoss) is training.py.
 """
 from dataclasses import dataclass
 from typing import Dict
