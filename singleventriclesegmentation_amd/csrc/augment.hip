@@ -209,19 +209,33 @@ struct VoxL {
   uint32_t out;      // byte offset of flip(q) in an output channel, kOutside when q is not inside
 };
 
+// All corner reads of the thread's voxels are issued before any product, so
+// a channel costs one LDS round trip per wave.
 template <int B>
-__device__ __forceinline__ float sample_box(const float* stage, const VoxL& x, int mode) {
+__device__ __forceinline__ void sample_box2(const float* stage, const VoxL (&vx)[kRotVox], int mode,
+                                            float (&val)[kRotVox]) {
   const char* sb = reinterpret_cast<const char*>(stage);
-  if (mode == 1) return reinterpret_cast<const float*>(sb + x.near)[B];
-  float acc = 0.f;
+  if (mode == 1) {
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const float* p = reinterpret_cast<const float*>(sb + x.pair[q]);
-    const float v0 = p[B], v1 = p[B + kLdsStride];
-    acc = acc + v0 * x.w[2 * q];
-    acc = acc + v1 * x.w[2 * q + 1];
+    for (int j = 0; j < kRotVox; ++j) val[j] = reinterpret_cast<const float*>(sb + vx[j].near)[B];
+    return;
   }
-  return acc;
+  float v[kRotVox][8];
+#pragma unroll
+  for (int j = 0; j < kRotVox; ++j)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float* p = reinterpret_cast<const float*>(sb + vx[j].pair[q]);
+      v[j][2 * q] = p[B];
+      v[j][2 * q + 1] = p[B + kLdsStride];
+    }
+#pragma unroll
+  for (int j = 0; j < kRotVox; ++j) {
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc = acc + v[j][k] * vx[j].w[k];
+    val[j] = acc;
+  }
 }
 
 // Position (unit, component) in the work-group's channel sequence.
@@ -276,14 +290,15 @@ __device__ __forceinline__ bool staged_step(const AugArgs& a, StagedLoop& st, co
   const int comp = st.cur.comp;
   const uint32_t comp_bytes = (uint32_t)(Nout * 4);
   const svs_rsrc dst = make_rsrc(c.dst, (c.vector ? 3u : 1u) * comp_bytes);
+  float val[kRotVox];
+  sample_box2<B>(stage, vx, c.mode, val);
 #pragma unroll
   for (int j = 0; j < kRotVox; ++j) {
-    const float val = sample_box<B>(stage, vx[j], c.mode);
     // Every component is rewritten on every path (a select, no branch).
-    st.vec[j][0] = comp == 0 ? val : st.vec[j][0];
-    st.vec[j][1] = comp == 1 ? val : st.vec[j][1];
-    st.vec[j][2] = comp == 2 ? val : st.vec[j][2];
-    if (!c.vector) bst(dst, vx[j].out, 0, val);
+    st.vec[j][0] = comp == 0 ? val[j] : st.vec[j][0];
+    st.vec[j][1] = comp == 1 ? val[j] : st.vec[j][1];
+    st.vec[j][2] = comp == 2 ? val[j] : st.vec[j][2];
+    if (!c.vector) bst(dst, vx[j].out, 0, val[j]);
   }
   if (c.vector && comp == 2) {
 #pragma unroll
