@@ -39,6 +39,8 @@ class Request:
     arrival: float                 # seconds on the server clock
     frames: Sequence[torch.Tensor]  # device volumes; a pair is (I0, I1)
     masks: Optional[Tuple[torch.Tensor, torch.Tensor, int]] = None  # (mask_first, mask_last, index_last)
+    # set by FlowServer.submit: the frames are ready once this event has passed
+    ready: Optional[torch.cuda.Event] = None
     # results
     flows: Optional[torch.Tensor] = None          # pair [Z,Y,X,3]; sequence [2(T-1),Z,Y,X,3]
     propagated: Optional[Tuple[torch.Tensor, torch.Tensor]] = None
@@ -102,11 +104,13 @@ class FlowServer:
                 return p
         return None
 
-    def _plan_for(self, shape: Shape, count: int) -> Tuple[Plan, tuple]:
+    def _plan_for(self, shape: Shape, count: int, out_bytes: int) -> Tuple[Plan, tuple]:
+        """The plan for (shape, count), reused when idle, with room made (by
+        evicting idle plans) for its workspace and the unit's outputs."""
         key = (shape, count)
         plan = self._take_idle(key)
+        self._make_room((0 if plan is not None else self._ws(key)) + out_bytes)
         if plan is None:
-            self._make_room(self._ws(key))
             plan = Plan(self.cfg, [shape] * count, device=self.device)
             self.used += plan.workspace_bytes
         return plan, key
@@ -158,6 +162,15 @@ class FlowServer:
 
     # ---------------------------------------------------------- requests
     def submit(self, req: Request):
+        """Queues a request.  Its frames (and masks) may still be being written
+        by work on the caller's current stream: the request's work waits for
+        that stream's work queued so far."""
+        if req.kind not in ("pair", "sequence"):
+            raise ValueError(f"request {req.rid}: kind must be 'pair' or 'sequence', got {req.kind!r}")
+        if req.kind == "sequence" and len(req.frames) < 2:
+            raise ValueError(f"request {req.rid}: a sequence needs at least 2 frames")
+        req.ready = torch.cuda.Event()
+        req.ready.record(torch.cuda.current_stream(self.device))
         self.pending.append(req)
 
     def _next_unit(self) -> Optional[Tuple[List[Request], Shape, int]]:
@@ -177,17 +190,33 @@ class FlowServer:
             return False
         reqs, shape, count = nxt
         nvox = shape[0] * shape[1] * shape[2]
-        out_bytes = count * nvox * 12 + (2 * nvox * 4 * 13 if reqs[0].kind == "sequence" else 0)
+        masks = reqs[0].masks if reqs[0].kind == "sequence" else None
+        out_bytes = count * nvox * 12 + (2 * nvox * 4 * (masks[2] + 1) if masks is not None else 0)
         if not self._fits(shape, count, out_bytes):
+            if not self.running:
+                # Nothing will be freed: the unit alone exceeds the budget.
+                raise RuntimeError(f"request {reqs[0].rid}: {count} solve(s) of {shape} need "
+                                   f"{self._ws((shape, count)) + out_bytes} bytes, over the "
+                                   f"{self.budget}-byte HBM budget")
             return False
         for r in reqs:
             self.pending.remove(r)
         slot = self.free_slots.popleft()
         stream = self.streams[slot]
-        plan, key = self._plan_for(shape, count)
+        plan, key = self._plan_for(shape, count, out_bytes)
         self.used += out_bytes
+        # The workspace and the outputs come from the caller's stream, so the
+        # slot stream starts after everything queued there so far (work that
+        # may still use the memory these blocks had before); it also waits for
+        # the stream each request was submitted from.
+        outputs = torch.empty((count,) + shape + (3,), dtype=torch.float32, device=self.device)
+        admitted = torch.cuda.Event()
+        admitted.record(torch.cuda.current_stream(self.device))
+        stream.wait_event(admitted)
+        for r in reqs:
+            if r.ready is not None:
+                stream.wait_event(r.ready)
         with torch.cuda.stream(stream):
-            outputs = torch.empty((count,) + shape + (3,), dtype=torch.float32, device=self.device)
             k = 0
             for r in reqs:
                 fr = r.frames
@@ -204,7 +233,7 @@ class FlowServer:
                 plan.bind(k, reqs[-1].frames[0], reqs[-1].frames[1], u_out=outputs[k])
                 k += 1
             plan.run(use_graph=self.use_graph)
-            if reqs[0].kind == "sequence":
+            if reqs[0].kind == "sequence" and reqs[0].masks is not None:
                 r = reqs[0]
                 T = len(r.frames)
                 m_first, m_last, j = r.masks
@@ -230,6 +259,12 @@ class FlowServer:
                 else:
                     r.flows = u.outputs[:r.solves]
                     k += r.solves
+                    if r.propagated is not None:
+                        # Made on the slot stream: their blocks must not go back
+                        # to it while the caller's stream may still read them.
+                        cur = torch.cuda.current_stream(self.device)
+                        for t in r.propagated:
+                            t.record_stream(cur)
                 r.done = now
                 self.completed.append(r)
             self.used -= u.nbytes

@@ -6,7 +6,7 @@ import pytest
 import torch
 
 from singleventriclesegmentation_amd import TVL13DOpticalFlow
-from singleventriclesegmentation_amd.serve import FlowServer, make_pool, poisson_requests, run_service
+from singleventriclesegmentation_amd.serve import FlowServer, Request, make_pool, poisson_requests, run_service
 from singleventriclesegmentation_amd.solver import SolverConfig, workspace_bytes
 
 pytestmark = pytest.mark.gpu
@@ -80,3 +80,36 @@ def test_warmed_and_tuned_plans_keep_results(dev):
     small.warm({sh: (pool[sh][0][1], pool[sh][0][0]) for sh in SHAPES}, frames=FRAMES, autotune=False)
     assert small.used <= small.budget
     small.close()
+
+
+def test_unit_over_budget_raises(dev):
+    """A unit that cannot fit even with every idle plan evicted and nothing
+    running is an error, not an endless wait."""
+    cfg = SolverConfig(num_scales=2, max_warps=1, max_outer_iterations=2)
+    pool = make_pool(SHAPES[:1], FRAMES, dev)
+    reqs = poisson_requests(0, 1, SHAPES[:1], pool, FRAMES, seed=9, p_pair=1.0)
+    server = FlowServer(cfg, device=dev, hbm_budget_bytes=1 << 16)
+    with pytest.raises(RuntimeError, match="HBM budget"):
+        run_service(server, reqs)
+    with pytest.raises(ValueError, match="kind"):
+        server.submit(Request(99, "triple", SHAPES[0], 0.0, reqs[0].frames))
+    server.close()
+
+
+def test_sequence_without_masks_from_fresh_frames(dev):
+    """Frames written on the caller's stream just before submit are what the
+    server solves; a sequence request without masks returns flows only."""
+    cfg = SolverConfig(num_scales=2, max_warps=2, max_outer_iterations=3)
+    pool = make_pool(SHAPES[:1], FRAMES, dev)
+    seq = pool[SHAPES[0]][0]
+    frames = [(f * 0.5 + 0.25).contiguous() for f in seq]   # produced on the current stream
+    req = Request(0, "sequence", SHAPES[0], 0.0, frames)
+    server = FlowServer(cfg, device=dev, slots=2)
+    run_service(server, [req])
+    assert req.propagated is None and req.flows.shape == (2 * (FRAMES - 1),) + SHAPES[0] + (3,)
+    solver = TVL13DOpticalFlow(2, 2, 3, 1.0, 0.025, 2, (0.5, 0.5, 0.25, 1.0), True, (5.0, 0.5), True, 1.0,
+                               "CUBIC_HERMITESPLINE", "MIRROR", "numDofs", (1, 1, 1), dev, use_graph=False)
+    for t in range(FRAMES - 1):
+        assert torch.equal(solver.compute(frames[t + 1], frames[t], None, None)[0], req.flows[t])
+        assert torch.equal(solver.compute(frames[t], frames[t + 1], None, None)[0], req.flows[FRAMES - 1 + t])
+    server.close()
