@@ -117,7 +117,52 @@ __device__ __forceinline__ Axis make_axis(float coord, float h) {
 // y, x, v) loads the NC values at an in-range voxel.  Loop order follows
 // interpolation_cubicHermiteSpline.cuh:482-525: x-pass per (dz,dy), y-pass per
 // dz, then the z-pass; each field is evaluated exactly as the scalar routine.
-template <int NC, bool MAYBE_HI, class Fetch>
+//
+// LOOP sets how many taps are in flight: 0 unrolls everything (64 taps may be
+// hoisted ahead of the first use), 1 runs the z-planes as a loop that is not
+// unrolled (16 taps), 2 also the rows of a plane (4 taps).  Fewer taps in
+// flight take fewer registers, so more waves share a SIMD; the partial
+// results reach their slots through selects, no arithmetic moves.
+template <int NC, bool MAYBE_HI, int LOOP, class Fetch>
+__device__ __forceinline__ void cubic_row(const Fetch& fetch, int z, int y, const int (&xs)[4], float lx,
+                                          float out[NC]) {
+  float v[4][NC];
+#pragma unroll
+  for (int dx = 0; dx < 4; ++dx) fetch(z, y, xs[dx], v[dx]);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) out[c] = cr_eval<MAYBE_HI>(v[0][c], v[1][c], v[2][c], v[3][c], lx);
+}
+
+template <int NC, bool MAYBE_HI, int LOOP, class Fetch>
+__device__ __forceinline__ void cubic_plane(const Fetch& fetch, int z, const int (&xs)[4], const int (&ys)[4],
+                                            float lx, float ly, float out[NC]) {
+  float by[NC][4];
+  if (LOOP >= 2) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) by[c][0] = by[c][1] = by[c][2] = by[c][3] = 0.0f;
+#pragma unroll 1
+    for (int dy = 0; dy < 4; ++dy) {
+      float r[NC];
+      cubic_row<NC, MAYBE_HI, LOOP>(fetch, z, ys[dy], xs, lx, r);
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) by[c][k] = k == dy ? r[c] : by[c][k];
+    }
+  } else {
+#pragma unroll
+    for (int dy = 0; dy < 4; ++dy) {
+      float r[NC];
+      cubic_row<NC, MAYBE_HI, LOOP>(fetch, z, ys[dy], xs, lx, r);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) by[c][dy] = r[c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) out[c] = cr_eval<MAYBE_HI>(by[c][0], by[c][1], by[c][2], by[c][3], ly);
+}
+
+template <int NC, bool MAYBE_HI, int LOOP, class Fetch>
 __device__ __forceinline__ void sample_cubic_impl(const Fetch& fetch, int NZ, int NY, int NX, Axis az, Axis ay,
                                                   Axis ax, int bc, float out[NC]) {
   int xs[4], ys[4];
@@ -128,20 +173,26 @@ __device__ __forceinline__ void sample_cubic_impl(const Fetch& fetch, int NZ, in
   }
   const float lx = ax.w + 1, ly = ay.w + 1, lz = az.w + 1;
   float bz[NC][4];
+  if (LOOP >= 1) {
 #pragma unroll
-  for (int dz = 0; dz < 4; ++dz) {
-    const int z = bidx(az.f - 1 + dz, NZ, bc);
-    float by[NC][4];
+    for (int c = 0; c < NC; ++c) bz[c][0] = bz[c][1] = bz[c][2] = bz[c][3] = 0.0f;
+#pragma unroll 1
+    for (int dz = 0; dz < 4; ++dz) {
+      float r[NC];
+      cubic_plane<NC, MAYBE_HI, LOOP>(fetch, bidx(az.f - 1 + dz, NZ, bc), xs, ys, lx, ly, r);
 #pragma unroll
-    for (int dy = 0; dy < 4; ++dy) {
-      float v[4][NC];
+      for (int c = 0; c < NC; ++c)
 #pragma unroll
-      for (int dx = 0; dx < 4; ++dx) fetch(z, ys[dy], xs[dx], v[dx]);
-#pragma unroll
-      for (int c = 0; c < NC; ++c) by[c][dy] = cr_eval<MAYBE_HI>(v[0][c], v[1][c], v[2][c], v[3][c], lx);
+        for (int k = 0; k < 4; ++k) bz[c][k] = k == dz ? r[c] : bz[c][k];
     }
+  } else {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) bz[c][dz] = cr_eval<MAYBE_HI>(by[c][0], by[c][1], by[c][2], by[c][3], ly);
+    for (int dz = 0; dz < 4; ++dz) {
+      float r[NC];
+      cubic_plane<NC, MAYBE_HI, LOOP>(fetch, bidx(az.f - 1 + dz, NZ, bc), xs, ys, lx, ly, r);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) bz[c][dz] = r[c];
+    }
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) out[c] = cr_eval<MAYBE_HI>(bz[c][0], bz[c][1], bz[c][2], bz[c][3], lz);
@@ -149,14 +200,14 @@ __device__ __forceinline__ void sample_cubic_impl(const Fetch& fetch, int NZ, in
 
 // The rounded-up segment (w + 1 == 2.0f, i.e. w = 1 - 2^-24) is rare: a
 // wavefront in which no lane meets it takes the select-free evaluation.
-template <int NC, class Fetch>
+template <int NC, class Fetch, int LOOP = 0>
 __device__ __forceinline__ void sample_cubic(const Fetch& fetch, int NZ, int NY, int NX, Axis az, Axis ay, Axis ax,
                                              int bc, float out[NC]) {
   const bool hi = ax.w + 1 >= 2.0f || ay.w + 1 >= 2.0f || az.w + 1 >= 2.0f;
   if (__any(hi))
-    sample_cubic_impl<NC, true>(fetch, NZ, NY, NX, az, ay, ax, bc, out);
+    sample_cubic_impl<NC, true, LOOP>(fetch, NZ, NY, NX, az, ay, ax, bc, out);
   else
-    sample_cubic_impl<NC, false>(fetch, NZ, NY, NX, az, ay, ax, bc, out);
+    sample_cubic_impl<NC, false, LOOP>(fetch, NZ, NY, NX, az, ay, ax, bc, out);
 }
 
 // interpolation_linear.cuh:249-295.
@@ -192,11 +243,11 @@ __device__ __forceinline__ int nearest_axis(float coord, float h, int N, int bc)
   return dist_f < dist_c ? bidx(f, N, bc) : bidx(f + 1, N, bc);
 }
 
-template <int NC, class Fetch>
+template <int NC, class Fetch, int LOOP = 0>
 __device__ __forceinline__ void sample(int ip, const Fetch& fetch, int NZ, int NY, int NX, float hZ, float hY,
                                        float hX, int bc, float cz, float cy, float cx, float out[NC]) {
   if (ip == IP_CUBIC) {
-    sample_cubic<NC>(fetch, NZ, NY, NX, make_axis(cz, hZ), make_axis(cy, hY), make_axis(cx, hX), bc, out);
+    sample_cubic<NC, Fetch, LOOP>(fetch, NZ, NY, NX, make_axis(cz, hZ), make_axis(cy, hY), make_axis(cx, hX), bc, out);
   } else if (ip == IP_LINEAR) {
     sample_linear<NC>(fetch, NZ, NY, NX, make_axis(cz, hZ), make_axis(cy, hY), make_axis(cx, hX), bc, out);
   } else {
