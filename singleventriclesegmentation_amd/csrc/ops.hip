@@ -211,28 +211,66 @@ __device__ void warp_backward_cubic(const float* __restrict__ u, const Acc& acc,
     zs[k] = bidx(az.f - 1 + k, NZ, bc);
   }
   auto U = [&](int z, int y, int x) { return u[((int64_t)z * NY + y) * NX + x]; };
-  float bz[4], by[4];
-  float byz[4][4];  // x-interpolated values per (dz, dy)
+  float bz[4] = {0.0f, 0.0f, 0.0f, 0.0f}, by[4];
+  float byz[4][4] = {};  // x-interpolated values per (dz, dy)
+  // Plane by plane (a loop that is not unrolled, results placed by selects):
+  // one plane's 16 taps in flight.
+#pragma unroll 1
   for (int dz = 0; dz < 4; ++dz) {
+    int z = zs[0];
+#pragma unroll
+    for (int k = 1; k < 4; ++k) z = dz == k ? zs[k] : z;
+    float r[4];
+#pragma unroll
     for (int dy = 0; dy < 4; ++dy) {
-      const int z = zs[dz], y = ys[dy];
-      byz[dz][dy] = cr_local(U(z, y, xs[0]), U(z, y, xs[1]), U(z, y, xs[2]), U(z, y, xs[3]), lx);
+      const int y = ys[dy];
+      r[dy] = cr_local(U(z, y, xs[0]), U(z, y, xs[1]), U(z, y, xs[2]), U(z, y, xs[3]), lx);
     }
-    bz[dz] = cr_local(byz[dz][0], byz[dz][1], byz[dz][2], byz[dz][3], ly);
+    const float b = cr_local(r[0], r[1], r[2], r[3], ly);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      bz[k] = dz == k ? b : bz[k];
+#pragma unroll
+      for (int dy = 0; dy < 4; ++dy) byz[k][dy] = dz == k ? r[dy] : byz[k][dy];
+    }
   }
   float gz[4];
   const float gwz = cr_local_backward(bz[0], bz[1], bz[2], bz[3], lz, fwd, gz);
   gphi[2] += gwz / hZ;
+  // A loop that is not unrolled: the taps are read again per plane instead
+  // of all 64 staying live from the forward pass (which spilled at the
+  // work-group's 128-VGPR budget).  The plane's values are picked by selects;
+  // the accumulation order is unchanged.
+#pragma unroll 1
   for (int dz = 0; dz < 4; ++dz) {
-    for (int dy = 0; dy < 4; ++dy) by[dy] = byz[dz][dy];
-    float gy[4];
-    const float gwy = cr_local_backward(by[0], by[1], by[2], by[3], ly, gz[dz], gy);
-    gphi[1] += gwy / hY;
+    float gzd = gz[0];
+    int z = zs[0];
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+      gzd = dz == k ? gz[k] : gzd;
+      z = dz == k ? zs[k] : z;
+    }
+#pragma unroll
     for (int dy = 0; dy < 4; ++dy) {
-      const int z = zs[dz], y = ys[dy];
+      by[dy] = byz[0][dy];
+#pragma unroll
+      for (int k = 1; k < 4; ++k) by[dy] = dz == k ? byz[k][dy] : by[dy];
+    }
+    float gy[4];
+    const float gwy = cr_local_backward(by[0], by[1], by[2], by[3], ly, gzd, gy);
+    gphi[1] += gwy / hY;
+#pragma unroll 1
+    for (int dy = 0; dy < 4; ++dy) {
+      int y = ys[0];
+      float gyd = gy[0];
+#pragma unroll
+      for (int k = 1; k < 4; ++k) {
+        y = dy == k ? ys[k] : y;
+        gyd = dy == k ? gy[k] : gyd;
+      }
       float gx[4];
       const float gwx = cr_local_backward(U(z, y, xs[0]), U(z, y, xs[1]), U(z, y, xs[2]), U(z, y, xs[3]), lx,
-                                          gy[dy], gx);
+                                          gyd, gx);
       gphi[0] += gwx / hX;
       for (int dx = 0; dx < 4; ++dx) acc(z, y, xs[dx], gx[dx]);
     }
@@ -280,7 +318,10 @@ __device__ void warp_backward_linear(const float* __restrict__ u, const Acc& acc
 // outside the window (larger displacements, far boundary reflections) go to
 // grad_u directly.  The reference also accumulates with float atomics, so
 // grad_u's low bits depend on the order in either case; grad_phi does not.
-constexpr int kAdjTX = 32, kAdjTY = 8, kAdjTZ = 4, kAdjM = 3;
+// A 16 x 16 x 1 tile (256 threads, 163 VGPRs, no spills, 29 KiB window) with a
+// 2-voxel margin measured fastest on 4 x 4 x 80^3 (2.9 ms; 32 x 8 x 4 with a
+// 3-voxel margin, 1024 threads capped at 128 VGPRs, spilled and took 4.0 ms).
+constexpr int kAdjTX = 16, kAdjTY = 16, kAdjTZ = 1, kAdjM = 2;
 constexpr int kAdjThreads = kAdjTX * kAdjTY * kAdjTZ;
 constexpr int kAdjWX = kAdjTX + 2 * (kAdjM + 2), kAdjWY = kAdjTY + 2 * (kAdjM + 2), kAdjWZ = kAdjTZ + 2 * (kAdjM + 2);
 constexpr int kAdjWin = kAdjWX * kAdjWY * kAdjWZ;
