@@ -69,6 +69,10 @@ class OpticalFlowProcessor:
         """processor.py:95-121 for one patient: the (T'-1, 3, X, Y, Z) flows."""
         img = self.frames(img_xyzt)
         times = self._times(tdia, tsys)
+        if len(times) < 2:
+            # tdia == tsys: no pair, an empty (0, 3, X, Y, Z) array as processor.py:106 allocates.
+            _, nz, ny, nx = img.shape
+            return np.zeros((0, 3, nx, ny, nz), dtype=np.float32)
         if self.batched and self.optflow.num_scales > 1:
             flows = self._pairs_batched(img, times)
         else:

@@ -70,3 +70,13 @@ def test_save_layout(dev, tmp_path):
     assert path.endswith("Child_3_forward_flow.npy")
     assert np.array_equal(np.load(path, allow_pickle=False), us)
     proc.close()
+
+
+@pytest.mark.parametrize("batched", [True, False])
+def test_equal_end_phases_give_no_pairs(dev, batched):
+    """tdia == tsys: compute_timepoints yields one frame, so the reference's loop
+    runs no pair and saves its empty (0, 3, X, Y, Z) array."""
+    proc = OpticalFlowProcessor(copy.deepcopy(FLOW_CFG), dev, batched=batched)
+    us = proc.process_patient(patient_xyzt(frames=4), 2, 2)
+    assert us.shape == (0, 3, 24, 20, 16) and us.dtype == np.float32
+    proc.close()
