@@ -379,9 +379,16 @@ def bench_serve(args, dev):
                             "solves_per_s": round(solves / span, 2), "requests": len(reqs), "solves": solves}
         log(args, f"rate {rate}: {res[f'{rate:g}']}")
     server.close()
+    # The plain per-request drop-in on the same 40-request set, its per-size plans
+    # created (and their graphs captured) before timing, as a long-running plain
+    # service would have them.
     plain = TVL13DOpticalFlow(4, 10, 50, 1.0, 0.025, 2, (0.5, 0.5, 0.25, 1.0), True, (5.0, 0.5), True, 1.0,
                               "CUBIC_HERMITESPLINE", "MIRROR", "numDofs", (1, 1, 1), dev)
-    reqs = poisson_requests(5.0, max(4, args.requests // 8), sizes, pool, args.frames, seed=4)
+    for sh in sizes:
+        plain.compute(pool[sh][0][1], pool[sh][0][0], None, None)
+    torch.cuda.synchronize()
+    log(args, "plain drop-in: one request at a time")
+    reqs = poisson_requests(5.0, args.requests, sizes, pool, args.frames, seed=4)
     lat, span = run_plain(plain, reqs)
     plain_res = {"rate": 5.0, "requests": len(reqs), "p50_s": round(float(np.percentile(lat, 50)), 3),
                  "p99_s": round(float(np.percentile(lat, 99)), 3),
