@@ -47,6 +47,15 @@ enum { IP_NEAREST = 0, IP_LINEAR = 1, IP_CUBIC = 2 };
 
 static inline float flr(float v) { return floorf(v); }
 static inline double flr(double v) { return floor(v); }
+// (int)floor(v) as the reference's CUDA build evaluates it (cvt.rzi.s32.f32:
+// NaN gives 0, no undefined behaviour), with magnitudes clamped to 2^30 so the
+// tap offsets added later cannot overflow; plain C++ leaves NaN and
+// out-of-range conversions undefined.
+template <class T> static inline int floor_int(T v) {
+  const T f = flr(v);
+  if (f != f) return 0;
+  return (int)std::min(std::max(f, (T)-1073741824.0), (T)1073741824.0);
+}
 static inline float sqrt_t(float v) { return sqrtf(v); }
 static inline double sqrt_t(double v) { return sqrt(v); }
 // The FP32 anisotropic weight uses the faithful expf/powf of svs_math.h, the
@@ -87,7 +96,7 @@ template <class T> static Mesh<T> make_mesh(int NZ, int NY, int NX, T LZ, T LY, 
 
 // interpolation_cubicHermiteSpline.cuh:20-56 — the local Catmull-Rom segment.
 template <class T> static inline T cr_local(const T* buf, T lc) {
-  const int f = (int)flr(lc);
+  const int f = floor_int(lc);
   const int c = f + 1, f1 = f - 1, c1 = c + 1;
   T u_f = 0, u_f1 = 0, u_c = 0, u_c1 = 0;
   if (f >= 0 && f < 4) u_f = buf[f];
@@ -110,7 +119,7 @@ template <class T> static inline T cr_local(const T* buf, T lc) {
 // grad_buf[k] = d out / d buf[k] * fwd, grad_w = d out / d w * fwd.  Taps the
 // reference would write outside [0,4) are dropped and untouched taps are 0.
 template <class T> static inline void cr_local_backward(const T* buf, T lc, T fwd, T* grad_buf, T* grad_w) {
-  const int f = (int)flr(lc);
+  const int f = floor_int(lc);
   const int c = f + 1, f1 = f - 1, c1 = c + 1;
   const T w = lc - (T)f;
   const T w2 = w * w;
@@ -145,11 +154,11 @@ template <class T> struct Field {
 // field), :695-739 (matrix field), :747-791 (CNN) are the same loop on a
 // different accessor.
 template <class T> static T sample_cubic(const Field<T>& u, T hZ, T hY, T hX, int bc, T cz, T cy, T cx) {
-  const int ix_f = (int)flr(cx / hX);
+  const int ix_f = floor_int(cx / hX);
   const T wx = cx / hX - (T)ix_f;
-  const int iy_f = (int)flr(cy / hY);
+  const int iy_f = floor_int(cy / hY);
   const T wy = cy / hY - (T)iy_f;
-  const int iz_f = (int)flr(cz / hZ);
+  const int iz_f = floor_int(cz / hZ);
   const T wz = cz / hZ - (T)iz_f;
   T bz[4], by[4], bx[4];
   for (int dz = -1; dz < 3; ++dz) {
@@ -166,11 +175,11 @@ template <class T> static T sample_cubic(const Field<T>& u, T hZ, T hY, T hX, in
 
 // interpolation_linear.cuh:249-295 (and the CNN copy :469-514).
 template <class T> static T sample_linear(const Field<T>& u, T hZ, T hY, T hX, int bc, T cz, T cy, T cx) {
-  const int ix_f = (int)flr(cx / hX); const int ix_c = ix_f + 1; const T wx = cx / hX - (T)ix_f;
+  const int ix_f = floor_int(cx / hX); const int ix_c = ix_f + 1; const T wx = cx / hX - (T)ix_f;
   const int xf = bidx(ix_f, u.NX, bc), xc = bidx(ix_c, u.NX, bc);
-  const int iy_f = (int)flr(cy / hY); const int iy_c = iy_f + 1; const T wy = cy / hY - (T)iy_f;
+  const int iy_f = floor_int(cy / hY); const int iy_c = iy_f + 1; const T wy = cy / hY - (T)iy_f;
   const int yf = bidx(iy_f, u.NY, bc), yc = bidx(iy_c, u.NY, bc);
-  const int iz_f = (int)flr(cz / hZ); const int iz_c = iz_f + 1; const T wz = cz / hZ - (T)iz_f;
+  const int iz_f = floor_int(cz / hZ); const int iz_c = iz_f + 1; const T wz = cz / hZ - (T)iz_f;
   const int zf = bidx(iz_f, u.NZ, bc), zc = bidx(iz_c, u.NZ, bc);
   const T u_fff = u.at(zf, yf, xf), u_ffc = u.at(zc, yf, xf), u_fcf = u.at(zf, yc, xf), u_fcc = u.at(zc, yc, xf);
   const T u_cff = u.at(zf, yf, xc), u_cfc = u.at(zc, yf, xc), u_ccf = u.at(zf, yc, xc), u_ccc = u.at(zc, yc, xc);
@@ -187,7 +196,7 @@ template <class T> static T sample_linear(const Field<T>& u, T hZ, T hY, T hX, i
 
 // interpolation_nearest.cuh:250-294: ties go to the upper neighbour.
 template <class T> static inline int nearest_axis(T c, T h, int N, int bc) {
-  const int f = (int)flr(c / h);
+  const int f = floor_int(c / h);
   const T dist_f = c / h - (T)f;
   const int cidx = f + 1;
   const T dist_c = (T)cidx - c / h;
@@ -254,9 +263,9 @@ template <class T> static void warp_cnn(const T* u, const T* phi, int B, int C, 
 // reference uses atomics; here voxels are visited in z,y,x order).
 template <class T> static void warp_backward_cubic_one(const Field<T>& u, T hZ, T hY, T hX, int bc,
                                                        T cz, T cy, T cx, T fwd, T* grad_u, size_t gstride, T* gphi) {
-  const int ix_f = (int)flr(cx / hX); const T wx = cx / hX - (T)ix_f;
-  const int iy_f = (int)flr(cy / hY); const T wy = cy / hY - (T)iy_f;
-  const int iz_f = (int)flr(cz / hZ); const T wz = cz / hZ - (T)iz_f;
+  const int ix_f = floor_int(cx / hX); const T wx = cx / hX - (T)ix_f;
+  const int iy_f = floor_int(cy / hY); const T wy = cy / hY - (T)iy_f;
+  const int iz_f = floor_int(cz / hZ); const T wz = cz / hZ - (T)iz_f;
   T bz[4], by[4], bx[4], gz[4], gy[4], gx[4], gw;
   for (int dz = -1; dz < 3; ++dz) {
     const int z = bidx(iz_f + dz, u.NZ, bc);
@@ -294,11 +303,11 @@ template <class T> static void warp_backward_cubic_one(const Field<T>& u, T hZ, 
 // interpolation_linear.cuh:297-363 and warpingOps.cu:390-415 (trilinear adjoint).
 template <class T> static void warp_backward_linear_one(const Field<T>& u, T hZ, T hY, T hX, int bc,
                                                         T cz, T cy, T cx, T fv, T* grad_u, size_t gs, T* gphi) {
-  const int ix_f = (int)flr(cx / hX); const int ix_c = ix_f + 1; const T wx = cx / hX - (T)ix_f;
+  const int ix_f = floor_int(cx / hX); const int ix_c = ix_f + 1; const T wx = cx / hX - (T)ix_f;
   const int xf = bidx(ix_f, u.NX, bc), xc = bidx(ix_c, u.NX, bc);
-  const int iy_f = (int)flr(cy / hY); const int iy_c = iy_f + 1; const T wy = cy / hY - (T)iy_f;
+  const int iy_f = floor_int(cy / hY); const int iy_c = iy_f + 1; const T wy = cy / hY - (T)iy_f;
   const int yf = bidx(iy_f, u.NY, bc), yc = bidx(iy_c, u.NY, bc);
-  const int iz_f = (int)flr(cz / hZ); const int iz_c = iz_f + 1; const T wz = cz / hZ - (T)iz_f;
+  const int iz_f = floor_int(cz / hZ); const int iz_c = iz_f + 1; const T wz = cz / hZ - (T)iz_f;
   const int zf = bidx(iz_f, u.NZ, bc), zc = bidx(iz_c, u.NZ, bc);
   const T u_fff = u.at(zf, yf, xf), u_ffc = u.at(zc, yf, xf), u_fcf = u.at(zf, yc, xf), u_fcc = u.at(zc, yc, xf);
   const T u_cff = u.at(zf, yf, xc), u_cfc = u.at(zc, yf, xc), u_ccf = u.at(zf, yc, xc), u_ccc = u.at(zc, yc, xc);

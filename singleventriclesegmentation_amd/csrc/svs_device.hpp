@@ -105,10 +105,20 @@ struct Axis {
   int f;    // floor(coord / h)
   float w;  // coord / h - f
 };
+// floor(v) as an int for every float: NaN gives 0, as the reference's CUDA
+// float-to-int conversion (and gfx950's) does, and magnitudes are clamped to
+// 2^30 so that the tap offsets added later cannot overflow (a coordinate that
+// far outside a volume lands on its boundary map either way).
+__device__ __forceinline__ int floor_int(float v) {
+  const float f = floorf(v);
+  if (f != f) return 0;
+  return (int)fminf(fmaxf(f, -1073741824.0f), 1073741824.0f);
+}
+
 __device__ __forceinline__ Axis make_axis(float coord, float h) {
   Axis a;
   const float v = coord / h;
-  a.f = (int)floorf(v);
+  a.f = floor_int(v);
   a.w = v - (float)a.f;
   return a;
 }
@@ -237,7 +247,7 @@ __device__ __forceinline__ void sample_linear(const Fetch& fetch, int NZ, int NY
 
 // interpolation_nearest.cuh:250-294 — ties go to the upper neighbour.
 __device__ __forceinline__ int nearest_axis(float coord, float h, int N, int bc) {
-  const int f = (int)floorf(coord / h);
+  const int f = floor_int(coord / h);
   const float dist_f = coord / h - (float)f;
   const float dist_c = (float)(f + 1) - coord / h;
   return dist_f < dist_c ? bidx(f, N, bc) : bidx(f + 1, N, bc);

@@ -100,6 +100,27 @@ def test_warp(O, dev, shape, bc, ip):
     assert np.array_equal(h(op.forwardVectorField(g(v, dev), g(phi, dev))), O.warp(v, phi, hs(m), ip, bc))
 
 
+@pytest.mark.parametrize("bc", BCS)
+@pytest.mark.parametrize("ip", IPS)
+def test_warp_with_nan_inputs(O, dev, ip, bc):
+    """NaN in the image and in the flow: the same NaN positions and the same
+    bits elsewhere as the oracle (a NaN coordinate floors to index 0, as the
+    reference's CUDA conversion gives)."""
+    r = np.random.default_rng(30 + ip + 3 * bc)
+    shape = (9, 10, 11)
+    m = mesh(shape)
+    u = r.standard_normal(shape).astype(np.float32)
+    u[4, 5, 6] = np.nan
+    phi = (r.standard_normal(shape + (3,)) * 1.5).astype(np.float32)
+    phi[2, 3, 4, :] = np.nan
+    phi[6, 1, 8, 1] = np.nan
+    op = OF.Warping3D(m, ip, bc)
+    assert np.array_equal(h(op.forward(g(u, dev), g(phi, dev))), O.warp(u, phi, hs(m), ip, bc), equal_nan=True)
+    cnn = OF.WarpingCNN3D(m, ip, bc)
+    got = h(cnn.forward(g(u[None, None], dev), g(phi[None], dev)))[0, 0]
+    assert np.array_equal(got, O.warp(u, phi, hs(m), ip, bc), equal_nan=True)
+
+
 @pytest.mark.parametrize("ip", IPS)
 def test_warp_cnn(O, dev, ip):
     r = np.random.default_rng(7)

@@ -77,6 +77,30 @@ def test_nearest_picks_the_closer_voxel_ties_up(O):
     assert list(w) == [0, 2, 3, 3, 4, 4]
 
 
+def test_nan_coordinates_follow_the_cuda_conversion(O):
+    """A NaN flow gives floor index 0 (CUDA's float-to-int conversion of NaN),
+    so NEAREST takes the upper neighbour, index 1, on every axis; LINEAR and
+    CUBIC give NaN at that voxel only.  A NaN image value spreads to the voxels
+    whose footprint holds it and nowhere else."""
+    r = np.random.default_rng(17)
+    shape = (9, 10, 11)
+    u = r.standard_normal(shape).astype(np.float32)
+    phi = (r.standard_normal(shape + (3,)) * 0.4).astype(np.float32)
+    phi[2, 3, 4, :] = np.nan
+    near = O.warp(u, phi, (1.0, 1.0, 1.0), 0, 1)
+    assert near[2, 3, 4] == u[1, 1, 1] and np.isfinite(near).all()
+    for ip in (1, 2):
+        out = O.warp(u, phi, (1.0, 1.0, 1.0), ip, 1)
+        assert np.isnan(out[2, 3, 4]) and np.isnan(out).sum() == 1
+    v = u.copy()
+    v[4, 5, 6] = np.nan
+    out = O.warp(v, np.zeros(shape + (3,), np.float32), (1.0, 1.0, 1.0), 2, 1)
+    # zero flow: each voxel's 4^3 footprint is itself and its neighbours at -1..+2
+    hit = np.zeros(shape, bool)
+    hit[4 - 2:4 + 2, 5 - 2:5 + 2, 6 - 2:6 + 2] = True
+    assert np.array_equal(np.isnan(out), hit)
+
+
 def test_vector_field_warp_is_componentwise(O):
     v = R.standard_normal((8, 9, 10, 3)).astype(np.float32)
     phi = R.uniform(-2, 2, (8, 9, 10, 3)).astype(np.float32)
