@@ -42,3 +42,18 @@ def test_workspace_size_and_validation():
     dims = (ctypes.c_int * 3)(8, 8, 8)  # level 3 would be 1^3
     assert _lib.lib().svs_tvl1_workspace_bytes(ctypes.byref(p), 1, dims) == 0
     assert b"pyramid level" in _lib.lib().svs_last_error()
+
+
+def test_plan_rejects_instances_beyond_32bit_offsets():
+    """The fused sweep addresses an instance's 9 dual fields with 32-bit byte
+    offsets: 36 N must stay below 2^32 (N = 119,304,647 voxels)."""
+    from singleventriclesegmentation_amd import _lib
+    from singleventriclesegmentation_amd.solver import SolverConfig
+    p, _w = SolverConfig().native()
+    ok = (ctypes.c_int * 3)(488, 488, 500)        # 119,072,000 voxels
+    assert _lib.lib().svs_tvl1_workspace_bytes(ctypes.byref(p), 1, ok) > 0
+    big = (ctypes.c_int * 3)(490, 490, 500)       # 120,050,000 voxels
+    assert _lib.lib().svs_tvl1_workspace_bytes(ctypes.byref(p), 1, big) == 0
+    assert b"exceeds" in _lib.lib().svs_last_error()
+    odd = (ctypes.c_int * 6)(16, 16, 16, 0, 16, 16)  # a zero-size second instance
+    assert _lib.lib().svs_tvl1_workspace_bytes(ctypes.byref(p), 2, odd) == 0
