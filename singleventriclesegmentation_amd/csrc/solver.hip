@@ -251,6 +251,112 @@ __global__ __launch_bounds__(kBlock) void k_prolong_state(const LevelDesc* __res
   }
 }
 
+// Tricubic coarse -> fine transfer in three separable passes.  The per-voxel
+// gather above evaluates, for every fine voxel, an x-pass on 16 coarse rows,
+// a y-pass per coarse plane and a z-pass.  The x-pass value of coarse row
+// (z', y') at fine x depends on nothing else, nor does the y-pass value of
+// coarse plane z' at fine (y, x), so computing each once and storing it gives
+// the same evaluations on the same values in the same order: the same bits,
+// for 12 instead of 64 taps per field and fine voxel.  cr_local is the
+// evaluation sample_cubic uses for any lane whose segment rounds up, and for
+// the others it gives the same bits as the select-free form.
+//   x-pass -> T1[f][z'][y'][x]  in the instance's J/SN/RG block (free here)
+//   y-pass -> T2[f][z'][y][x]   in the fine level's second dual buffer
+//   z-pass -> u, p of the fine level, u scaled per component
+constexpr int kProlongFields = 12;   // u (3 components) and p (9)
+
+__device__ __forceinline__ const float* state_field(const LevelDesc& c, int parity, int f) {
+  return f < 3 ? c.u[parity] + (int64_t)f * c.N : c.p[parity] + (int64_t)(f - 3) * c.N;
+}
+
+__global__ __launch_bounds__(kBlock) void k_prolong_x(const LevelDesc* __restrict__ coarse,
+                                                      const LevelDesc* __restrict__ fine,
+                                                      const int* __restrict__ block_start, int ninst, int parity_c,
+                                                      int bc) {
+  const int inst = find_instance(block_start, ninst, (int)blockIdx.x);
+  const LevelDesc& c = coarse[inst];
+  const LevelDesc& d = fine[inst];
+  const int64_t P1 = (int64_t)c.NZ * c.NY * d.NX;
+  const int64_t i = (int64_t)(blockIdx.x - block_start[inst]) * kBlock + threadIdx.x;
+  if (i >= P1) return;
+  const int x = (int)(i % d.NX);
+  const int64_t row = (i / d.NX) * c.NX;   // coarse row (z', y')
+  const float cxp = (float)x * d.hX;
+  const float cx = cxp * c.LX / d.LX;
+  const Axis ax = make_axis(cx, c.hX);
+  int xs[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) xs[k] = bidx(ax.f - 1 + k, c.NX, bc);
+  const float lx = ax.w + 1;
+  float* T1 = reinterpret_cast<float*>(d.J);
+#pragma unroll 4
+  for (int f = 0; f < kProlongFields; ++f) {
+    const float* s = state_field(c, parity_c, f) + row;
+    T1[f * P1 + i] = cr_local(s[xs[0]], s[xs[1]], s[xs[2]], s[xs[3]], lx);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_prolong_y(const LevelDesc* __restrict__ coarse,
+                                                      const LevelDesc* __restrict__ fine,
+                                                      const int* __restrict__ block_start, int ninst, int bc) {
+  const int inst = find_instance(block_start, ninst, (int)blockIdx.x);
+  const LevelDesc& c = coarse[inst];
+  const LevelDesc& d = fine[inst];
+  const int64_t P1 = (int64_t)c.NZ * c.NY * d.NX;
+  const int64_t P2 = (int64_t)c.NZ * d.NY * d.NX;
+  const int64_t i = (int64_t)(blockIdx.x - block_start[inst]) * kBlock + threadIdx.x;
+  if (i >= P2) return;
+  const int x = (int)(i % d.NX);
+  const int64_t r = i / d.NX;
+  const int y = (int)(r % d.NY);
+  const int zc = (int)(r / d.NY);
+  const float cyp = (float)y * d.hY;
+  const float cy = cyp * c.LY / d.LY;
+  const Axis ay = make_axis(cy, c.hY);
+  int64_t t[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) t[k] = ((int64_t)zc * c.NY + bidx(ay.f - 1 + k, c.NY, bc)) * d.NX + x;
+  const float ly = ay.w + 1;
+  const float* T1 = reinterpret_cast<const float*>(d.J);
+  float* T2 = d.p[1];
+#pragma unroll 4
+  for (int f = 0; f < kProlongFields; ++f) {
+    const float* s = T1 + f * P1;
+    T2[f * P2 + i] = cr_local(s[t[0]], s[t[1]], s[t[2]], s[t[3]], ly);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_prolong_z(const LevelDesc* __restrict__ coarse,
+                                                      const LevelDesc* __restrict__ fine,
+                                                      const int* __restrict__ block_start, int ninst, int bc) {
+  SVS_BATCH_PROLOGUE(fine);
+  (void)y; (void)x;
+  const LevelDesc& c = coarse[inst_];
+  const int64_t Nf = d.N;
+  const int64_t P2 = (int64_t)c.NZ * d.NY * d.NX;
+  const int64_t plane = (int64_t)d.NY * d.NX;
+  const int64_t yx = i - (int64_t)z * plane;
+  const float czp = (float)z * d.hZ;
+  const float cz = czp * c.LZ / d.LZ;
+  const Axis az = make_axis(cz, c.hZ);
+  int64_t t[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) t[k] = (int64_t)bidx(az.f - 1 + k, c.NZ, bc) * plane + yx;
+  const float lz = az.w + 1;
+  const float* T2 = d.p[1];
+  const float up[3] = {d.upX, d.upY, d.upZ};
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const float* s = T2 + f * P2;
+    d.u[0][f * Nf + i] = cr_local(s[t[0]], s[t[1]], s[t[2]], s[t[3]], lz) * up[f];
+  }
+#pragma unroll 3
+  for (int f = 3; f < kProlongFields; ++f) {
+    const float* s = T2 + f * P2;
+    d.p[0][(f - 3) * Nf + i] = cr_local(s[t[0]], s[t[1]], s[t[2]], s[t[3]], lz);
+  }
+}
+
 // SoA state -> the caller's [Z,Y,X,3] flow and [Z,Y,X,3,3] dual.
 __global__ __launch_bounds__(kBlock) void k_store_output(const LevelDesc* __restrict__ descs, const int* __restrict__ block_start, int ninst,
                                int parity) {
@@ -513,6 +619,12 @@ struct LevelPlan {
   LevelDesc* dev = nullptr;      // table in the workspace
   int* block_start = nullptr;    // per instance, + total
   int nblocks = 0;
+  // Separable prolongation into this level from the next coarser one:
+  // block tables of the x-pass (coarse z, coarse y, fine x) and the y-pass
+  // (coarse z, fine y, fine x) domains.
+  int* px_start = nullptr;
+  int* py_start = nullptr;
+  int px_blocks = 0, py_blocks = 0;
   SweepTile* tiles = nullptr;
   int ntiles = 0;
   int tile_capacity = 0;
@@ -544,6 +656,10 @@ struct svs_plan {
 namespace svs {
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Domains of the separable prolongation's x-pass and y-pass.
+static int64_t prolong_x_domain(const LevelDesc& c, const LevelDesc& f) { return (int64_t)c.NZ * c.NY * f.NX; }
+static int64_t prolong_y_domain(const LevelDesc& c, const LevelDesc& f) { return (int64_t)c.NZ * f.NY * f.NX; }
 
 // numDofs meshes use L = N - 1 (flow_utils.py:48-71); h = L / (N - 1) in float.
 static void level_geometry(const svs_tvl1_params& P, int NZ, int NY, int NX, LevelDesc& d) {
@@ -736,9 +852,11 @@ static int plan_layout(svs_plan* pl, char* base, size_t* total_out) {
   for (int k = 0; k < pl->ninst; ++k) {
     int nz = pl->dims[3 * k], ny = pl->dims[3 * k + 1], nx = pl->dims[3 * k + 2];
     const int64_t N0 = (int64_t)nz * ny * nx;
-    float4* J = (float4*)take(16 * N0);
-    float4* SN = (float4*)take(16 * N0);
-    float4* RG = (float4*)take(16 * N0);
+    // One block, so that the separable prolongation can use all of it for its
+    // x-pass (<= 12 * 0.625^2 floats per fine voxel; free while it runs).
+    float4* J = (float4*)take(48 * N0);
+    float4* SN = J + N0;
+    float4* RG = J + 2 * N0;
     for (int s = 0; s < S; ++s) {
       if (s > 0) { nz = (nz + 1) / 2; ny = (ny + 1) / 2; nx = (nx + 1) / 2; }
       LevelDesc& d = pl->levels[s].host[k];
@@ -762,6 +880,8 @@ static int plan_layout(svs_plan* pl, char* base, size_t* total_out) {
     choose_sweep_shape(P, L, pl->ninst);
     L.dev = (LevelDesc*)take(sizeof(LevelDesc) * pl->ninst);
     L.block_start = (int*)take(sizeof(int) * (pl->ninst + 1));
+    L.px_start = (int*)take(sizeof(int) * (pl->ninst + 1));
+    L.py_start = (int*)take(sizeof(int) * (pl->ninst + 1));
     L.ntiles = count_tiles(L, L.bx, L.by, L.cz);
     // Room for any shape the autotuner may try.
     int cap = L.ntiles;
@@ -779,6 +899,12 @@ static int plan_layout(svs_plan* pl, char* base, size_t* total_out) {
     }
     L.nblocks = nb;
     L.unit_h = unit;
+    L.px_blocks = L.py_blocks = 0;
+    if (s + 1 < S)
+      for (int k = 0; k < pl->ninst; ++k) {
+        L.px_blocks += grid_for(prolong_x_domain(pl->levels[s + 1].host[k], L.host[k]), kBlock);
+        L.py_blocks += grid_for(prolong_y_domain(pl->levels[s + 1].host[k], L.host[k]), kBlock);
+      }
   }
   if (P.use_blur) pl->blur_dev = (float*)take(sizeof(float) * pl->blur_w.size());
   *total_out = align_up(off, 256);
@@ -865,6 +991,20 @@ static int upload_static(svs_plan* pl, hipStream_t st) {
     bs[pl->ninst] = acc;
     SVS_HIP(hipMemcpyAsync(L.block_start, bs.data(), sizeof(int) * bs.size(), hipMemcpyHostToDevice, st));
     SVS_HIP(hipStreamSynchronize(st));
+    if (s + 1 < pl->S) {
+      std::vector<int> bx(pl->ninst + 1), by(pl->ninst + 1);
+      int ax = 0, ay = 0;
+      for (int k = 0; k < pl->ninst; ++k) {
+        const LevelDesc& c = pl->levels[s + 1].host[k];
+        bx[k] = ax; ax += grid_for(prolong_x_domain(c, L.host[k]), kBlock);
+        by[k] = ay; ay += grid_for(prolong_y_domain(c, L.host[k]), kBlock);
+      }
+      bx[pl->ninst] = ax;
+      by[pl->ninst] = ay;
+      SVS_HIP(hipMemcpyAsync(L.px_start, bx.data(), sizeof(int) * bx.size(), hipMemcpyHostToDevice, st));
+      SVS_HIP(hipMemcpyAsync(L.py_start, by.data(), sizeof(int) * by.size(), hipMemcpyHostToDevice, st));
+      SVS_HIP(hipStreamSynchronize(st));
+    }
     if (int rc = upload_tiles(pl, s, st)) return rc;
   }
   if (pl->P.use_blur) {
@@ -999,7 +1139,15 @@ static int enqueue_solve(svs_plan* pl, hipStream_t st) {
     }
     if (s > 0) {
       LevelPlan& F = pl->levels[s - 1];
-      k_prolong_state<<<F.nblocks, kBlock, 0, st>>>(L.dev, F.dev, F.block_start, n, parity, ip, bc);
+      if (ip == IP_CUBIC) {
+        k_prolong_x<<<F.px_blocks, kBlock, 0, st>>>(L.dev, F.dev, F.px_start, n, parity, bc);
+        SVS_LAUNCHED();
+        k_prolong_y<<<F.py_blocks, kBlock, 0, st>>>(L.dev, F.dev, F.py_start, n, bc);
+        SVS_LAUNCHED();
+        k_prolong_z<<<F.nblocks, kBlock, 0, st>>>(L.dev, F.dev, F.block_start, n, bc);
+      } else {
+        k_prolong_state<<<F.nblocks, kBlock, 0, st>>>(L.dev, F.dev, F.block_start, n, parity, ip, bc);
+      }
       SVS_LAUNCHED();
     }
   }
