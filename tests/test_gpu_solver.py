@@ -62,6 +62,20 @@ def test_fused_matches_oracle_small(O, dev, boundary, aniso):
     assert np.array_equal(u, uo) and np.array_equal(p, po)
 
 
+@pytest.mark.parametrize("boundary", ["MIRROR", "NEAREST", "REFLECT"])
+def test_four_levels_down_to_four_voxels_match_oracle(O, dev, boundary):
+    """Odd level sizes all the way to the 4-voxel minimum (25x30x27 -> 13x15x14 ->
+    7x8x7 -> 4x4x4): restriction, the separable prolongation between odd sizes
+    and every boundary rule, bit for bit."""
+    shape = (25, 30, 27)
+    I0, I1, _ = pair(shape, seed=21)
+    S, W, K = 4, 2, 6
+    solver = make_solver(S, W, K, boundary_type=boundary)
+    u, p = solver.compute(torch.from_numpy(I0).to(dev), torch.from_numpy(I1).to(dev), None, None)
+    uo, po = oracle_solve(O, I0, I1, S, W, K, boundary=boundary)
+    assert np.array_equal(u.cpu().numpy(), uo) and np.array_equal(p.cpu().numpy(), po)
+
+
 @pytest.mark.parametrize("interpolation,length_type,length", [
     ("NEAREST", "numDofs", (1, 1, 1)), ("LINEAR", "numDofs", (1, 1, 1)),
     ("CUBIC_HERMITESPLINE", "fixed", (30.0, 22.5, 17.0)), ("LINEAR", "fixed", (9.0, 31.0, 20.0))])
