@@ -742,6 +742,11 @@ static std::vector<SweepShape> tile_candidates(const LevelPlan& L) {
   int maxNX = 0, maxNY = 0;
   for (auto& d : L.host) { maxNX = std::max(maxNX, d.NX); maxNY = std::max(maxNY, d.NY); }
   std::vector<int> txs = {maxNX, (maxNX + 1) / 2, (maxNX + 2) / 3, (maxNX + 3) / 4, 16, 8};
+  // Uneven batches: every instance's own width and its half as well, so that a
+  // policy can give full rows to one size while splitting wider ones.
+  for (auto& d : L.host)
+    for (int tx : {d.NX, (d.NX + 1) / 2})
+      if (std::find(txs.begin(), txs.end(), tx) == txs.end()) txs.push_back(tx);
   std::vector<int> tys = {2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 20};
   std::vector<SweepShape> out;
   for (int tx : txs) {
