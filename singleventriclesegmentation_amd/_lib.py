@@ -4,6 +4,7 @@ There is no fallback: if the library is missing or cannot be loaded, every
 operator raises.  torch is imported first so that the library binds to the HIP
 runtime torch already loaded (both carry the soname libamdhip64.so.7).
 """
+import contextlib
 import ctypes
 import os
 
@@ -108,8 +109,28 @@ def check(rc):
         raise RuntimeError(msg or f"libsvsflow call failed with status {rc}")
 
 
+# torch's raw current-stream query (what its own generated code calls); it
+# skips building a Stream object per call, which dominated the host time of
+# the operator-by-operator path.  Older builds without it use the public API.
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def stream_ptr(device=None):
+    """torch's current stream on ``device`` (default: the current device) as a hipStream_t."""
+    if _raw_stream is not None:
+        idx = torch.cuda.current_device() if device is None else torch.cuda._utils._get_device_index(device, True)
+        return ctypes.c_void_p(_raw_stream(idx))
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def on_device(device):
+    """Context that makes ``device`` current for the launches inside it; a
+    no-op when it already is (the common case, and a measurable share of an
+    operator call's host time otherwise)."""
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    if idx == torch.cuda.current_device():
+        return contextlib.nullcontext()
+    return torch.cuda.device(idx)
 
 
 def ptr(t):

@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check, ptr, stream_ptr
+from ._lib import check, on_device, ptr, stream_ptr
 
 
 class BoundaryType(enum.IntEnum):
@@ -126,14 +126,14 @@ class Nabla3D_CD(_MeshOp):
         self._bc = _enum(boundary, BoundaryType, "boundary type")
 
     def _fwd(self, b, C):
-        with torch.cuda.device(b.device):
+        with on_device(b.device):
             out = torch.empty(tuple(b.shape) + (3,), dtype=torch.float32, device=b.device)
             check(_lib.lib().svs_nabla3d_forward(ptr(b), C, *b.shape[:3], *self._h, self._bc, ptr(out),
                                                  stream_ptr()))
         return out
 
     def _bwd(self, p, C):
-        with torch.cuda.device(p.device):
+        with on_device(p.device):
             out = torch.empty(tuple(p.shape[:-1]), dtype=torch.float32, device=p.device)
             check(_lib.lib().svs_nabla3d_backward(ptr(p), C, *p.shape[:3], *self._h, self._bc, ptr(out),
                                                   stream_ptr()))
@@ -180,7 +180,7 @@ class AnisotropicNabla3D(_MeshOp):
         _check(g, "NablaInputImage", 4)
         if g.shape[3] != 3:
             raise RuntimeError("Expected last dimension 3")
-        with torch.cuda.device(g.device):
+        with on_device(g.device):
             s = torch.empty(tuple(g.shape[:3]), dtype=torch.float32, device=g.device)
             n, t1, t2 = (torch.empty_like(g) for _ in range(3))
             check(_lib.lib().svs_aniso_tangents(ptr(g), s.numel(), self._alpha, self._beta, ptr(s), ptr(n), ptr(t1),
@@ -199,7 +199,7 @@ class AnisotropicNabla3D(_MeshOp):
         _shape_eq(scalars, sp, "scalars")
         for t, nm in ((normals, "normals"), (tangents1, "tangents1"), (tangents2, "tangents2")):
             _shape_eq(t, sp + (3,), nm)
-        with torch.cuda.device(b.device):
+        with on_device(b.device):
             out = torch.empty_like(b)
             check(_lib.lib().svs_aniso_apply(ptr(b), ptr(scalars), ptr(normals), ptr(tangents1), ptr(tangents2),
                                              scalars.numel(), ptr(out), stream_ptr()))
@@ -230,7 +230,7 @@ class Warping3D(_MeshOp):
         """u [Z,Y,X], phi [Z,Y,X,3] -> [Z,Y,X]."""
         _check(u, "u", 3)
         self._check_phi(u, phi)
-        with torch.cuda.device(u.device):
+        with on_device(u.device):
             out = torch.empty_like(u)
             check(_lib.lib().svs_warp3d(ptr(u), 1, ptr(phi), *u.shape, *self._h, self._ip, self._bc, ptr(out),
                                         stream_ptr()))
@@ -240,7 +240,7 @@ class Warping3D(_MeshOp):
         """u [Z,Y,X,C], phi [Z,Y,X,3] -> [Z,Y,X,C]."""
         _check(u, "u", 4)
         self._check_phi(u, phi)
-        with torch.cuda.device(u.device):
+        with on_device(u.device):
             out = torch.empty_like(u)
             check(_lib.lib().svs_warp3d(ptr(u), u.shape[3], ptr(phi), *u.shape[:3], *self._h, self._ip, self._bc,
                                         ptr(out), stream_ptr()))
@@ -252,7 +252,7 @@ class Warping3D(_MeshOp):
         self._check_phi(u, phi)
         _check(forward_out, "forward_out", 3)
         _shape_eq(forward_out, u.shape, "forward_out")
-        with torch.cuda.device(u.device):
+        with on_device(u.device):
             gu = torch.empty_like(u)
             gphi = torch.empty_like(phi)
             check(_lib.lib().svs_warp3d_backward(ptr(u), ptr(phi), ptr(forward_out), 1, 1, *u.shape, *self._h,
@@ -281,7 +281,7 @@ class WarpingCNN3D(_MeshOp):
 
     def forward(self, u, phi):
         self._check_args(u, phi)
-        with torch.cuda.device(u.device):
+        with on_device(u.device):
             out = torch.empty_like(u)
             check(_lib.lib().svs_warp3d_cnn(ptr(u), ptr(phi), u.shape[0], u.shape[1], *u.shape[2:], *self._h,
                                             self._ip, self._bc, ptr(out), stream_ptr()))
@@ -291,7 +291,7 @@ class WarpingCNN3D(_MeshOp):
         self._check_args(u, phi)
         _check(grad_out, "grad_out", 5)
         _shape_eq(grad_out, u.shape, "grad_out")
-        with torch.cuda.device(u.device):
+        with on_device(u.device):
             gu = torch.empty_like(u)
             gphi = torch.empty_like(phi)
             check(_lib.lib().svs_warp3d_backward(ptr(u), ptr(phi), ptr(grad_out), u.shape[0], u.shape[1],
@@ -318,7 +318,7 @@ class Prolongation3D:
         # (prolongationOps.cu:1000-1010); a tensor of another size would be
         # sampled with a foreign spacing, so it is rejected here.
         _shape_eq(u, tuple(self._src_N) + trailing, "u")
-        with torch.cuda.device(u.device):
+        with on_device(u.device):
             out = torch.empty(tuple(self._dst_N) + trailing, dtype=torch.float32, device=u.device)
             check(_lib.lib().svs_prolongate3d(ptr(u), C, *u.shape[:3], *self._src_L, *self._dst_N, *self._dst_L,
                                               self._ip, self._bc, ptr(out), stream_ptr()))
@@ -351,7 +351,7 @@ def TVL1OF3D_proxPrimal(primalVariable, primalStepSize_tau, primalFctWeight_Matc
     _shape_eq(primalVariable, sp + (3,), "primalVariable")
     _shape_eq(rho, sp, "rho")
     _shape_eq(I1_warped_grad, sp + (3,), "I1_warped_grad")
-    with torch.cuda.device(primalVariable.device):
+    with on_device(primalVariable.device):
         out = torch.empty_like(primalVariable)
         check(_lib.lib().svs_prox_primal(ptr(primalVariable), float(np.float32(primalStepSize_tau)),
                                          float(np.float32(primalFctWeight_Matching)), ptr(rho), ptr(I1_warped_grad),
@@ -364,7 +364,7 @@ def TVL1OF3D_proxDual(dualVariable, dualStepSize_sigma, dualFctWeight_TV, meshIn
     _check(dualVariable, "dualVariable", 5)
     sp = (meshInfo.getNZ(), meshInfo.getNY(), meshInfo.getNX())
     _shape_eq(dualVariable, sp + (3, 3), "dualVariable")
-    with torch.cuda.device(dualVariable.device):
+    with on_device(dualVariable.device):
         out = torch.empty_like(dualVariable)
         check(_lib.lib().svs_prox_dual(ptr(dualVariable), float(np.float32(dualStepSize_sigma)),
                                        float(np.float32(dualFctWeight_TV)), dualVariable.numel() // 9, ptr(out),
