@@ -502,24 +502,33 @@ struct OddEvenMergeNet {
 };
 
 // 3x3x3: the 27 window values in registers through the merge network; only
-// the comparators that feed rank 13 survive dead-code elimination.
+// the comparators that feed rank 13 survive dead-code elimination.  A window
+// offset of at most one voxel needs a single reflection (scipy's 'reflect':
+// -1 -> 0, N -> N - 1), two selects instead of reflect_index's modulo, and the
+// element index is split in 32-bit arithmetic when the launch allows it.
+__device__ __forceinline__ int reflect1(int i, int N) { return i < 0 ? -i - 1 : (i >= N ? 2 * N - 1 - i : i); }
+
+template <typename I>
 __global__ __launch_bounds__(kBlock) void k_median3(const float* __restrict__ in, int NZ, int NY, int NX, int C,
                                                     int64_t total, float* __restrict__ out) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= total) return;
-  const int c = (int)(t % C);
-  const int64_t i = t / C;
-  const int64_t N = (int64_t)NZ * NY * NX;
-  const int64_t b = i / N;
-  int z, y, x;
-  unflatten(i - b * N, NY, NX, z, y, x);
-  const float* src = in + b * N * C + c;
+  const I t = (I)((int64_t)blockIdx.x * kBlock + threadIdx.x);
+  if ((int64_t)t >= total) return;
+  const I c = t % (I)C;
+  const I i = t / (I)C;
+  const I N = (I)NZ * NY * NX;
+  const I b = i / N;
+  const I r = i - b * N;
+  const int x = (int)(r % NX);
+  const I zy = r / NX;
+  const int y = (int)(zy % NY);
+  const int z = (int)(zy / NY);
+  const float* src = in + (int64_t)b * N * C + c;
   int zs[3], ys[3], xs[3];
 #pragma unroll
   for (int o = 0; o < 3; ++o) {
-    zs[o] = reflect_index(z + o - 1, NZ);
-    ys[o] = reflect_index(y + o - 1, NY);
-    xs[o] = reflect_index(x + o - 1, NX);
+    zs[o] = reflect1(z + o - 1, NZ);
+    ys[o] = reflect1(y + o - 1, NY);
+    xs[o] = reflect1(x + o - 1, NX);
   }
   float v[27];
 #pragma unroll
@@ -528,7 +537,7 @@ __global__ __launch_bounds__(kBlock) void k_median3(const float* __restrict__ in
     for (int e = 0; e < 3; ++e)
 #pragma unroll
       for (int f = 0; f < 3; ++f)
-        v[(a * 3 + e) * 3 + f] = src[(((int64_t)zs[a] * NY + ys[e]) * NX + xs[f]) * C];
+        v[(a * 3 + e) * 3 + f] = src[(((I)zs[a] * NY + ys[e]) * NX + xs[f]) * C];
   constexpr OddEvenMergeNet<27> net;
 #pragma unroll
   for (int q = 0; q < net.n; ++q) {
@@ -750,8 +759,10 @@ int svs_median3d(const float* in, int B, int NZ, int NY, int NX, int C, int ks, 
   SVS_REQUIRE_DIMS(NZ, NY, NX);
   const int64_t total = (int64_t)B * NZ * NY * NX * C;
   SVS_REQUIRE(total < ((int64_t)1 << 38), "apply_median_filter: too many elements");
-  if (ks == 3)
-    k_median3<<<grid_for(total, kBlock), kBlock, 0, (hipStream_t)stream>>>(in, NZ, NY, NX, C, total, out);
+  if (ks == 3 && total < ((int64_t)1 << 31))
+    k_median3<int32_t><<<grid_for(total, kBlock), kBlock, 0, (hipStream_t)stream>>>(in, NZ, NY, NX, C, total, out);
+  else if (ks == 3)
+    k_median3<int64_t><<<grid_for(total, kBlock), kBlock, 0, (hipStream_t)stream>>>(in, NZ, NY, NX, C, total, out);
   else
     k_median_any<<<grid_for(total, kBlock), kBlock, 0, (hipStream_t)stream>>>(in, NZ, NY, NX, C, ks, total, out);
   SVS_LAUNCHED();
