@@ -118,7 +118,10 @@ _raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
 def stream_ptr(device=None):
     """torch's current stream on ``device`` (default: the current device) as a hipStream_t."""
     if _raw_stream is not None:
-        idx = torch.cuda.current_device() if device is None else torch.cuda._utils._get_device_index(device, True)
+        if device is None or (isinstance(device, torch.device) and device.index is None):
+            idx = torch.cuda.current_device()
+        else:
+            idx = device.index if isinstance(device, torch.device) else int(device)
         return ctypes.c_void_p(_raw_stream(idx))
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
