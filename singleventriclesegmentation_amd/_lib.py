@@ -116,14 +116,15 @@ _raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
 
 
 def stream_ptr(device=None):
-    """torch's current stream on ``device`` (default: the current device) as a hipStream_t."""
+    """torch's current stream on ``device`` (default: the current device) as a hipStream_t
+    address for a c_void_p argument."""
     if _raw_stream is not None:
         if device is None or (isinstance(device, torch.device) and device.index is None):
             idx = torch.cuda.current_device()
         else:
             idx = device.index if isinstance(device, torch.device) else int(device)
-        return ctypes.c_void_p(_raw_stream(idx))
-    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+        return _raw_stream(idx)
+    return torch.cuda.current_stream(device).cuda_stream
 
 
 def on_device(device):
@@ -137,4 +138,6 @@ def on_device(device):
 
 
 def ptr(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+    """A tensor's device address for a c_void_p argument (None passes NULL);
+    ctypes converts the plain int itself."""
+    return t.data_ptr() if t is not None else None
