@@ -206,7 +206,8 @@ typedef struct svs_plan svs_plan;
 size_t svs_tvl1_workspace_bytes(const svs_tvl1_params* params, int ninst, const int* dims);
 
 /* Builds a plan inside `workspace` (>= svs_tvl1_workspace_bytes).  Uploads the
- * static tables on `stream` and synchronises it once. */
+ * static tables on `stream` and synchronises it (plan creation blocks; runs
+ * do not). */
 int svs_tvl1_plan_create(const svs_tvl1_params* params, int ninst, const int* dims, void* workspace,
                          size_t ws_bytes, void* stream, svs_plan** plan);
 
@@ -217,14 +218,20 @@ int svs_tvl1_plan_create(const svs_tvl1_params* params, int ninst, const int* di
 int svs_tvl1_plan_bind(svs_plan* plan, int k, const float* I0, const float* I1, const float* u_in,
                        const float* p_in, float* u_out, float* p_out);
 
-/* Enqueues every instance's solve on `stream`.  use_graph = 1 captures the
- * fixed level x warp x iteration schedule as a hipGraph on first use and
- * replays it; re-binding instances keeps the graph.  If bindings changed
- * since the last run, the descriptor tables are re-uploaded and `stream` is
- * synchronised once. */
+/* Enqueues every instance's solve on `stream` and returns without waiting
+ * for it.  use_graph = 1 captures the fixed level x warp x iteration schedule
+ * as a hipGraph on first use and replays it; re-binding instances keeps the
+ * graph.  If bindings changed since the last run, the descriptor tables are
+ * copied in stream order from a ring of 4 pinned staging slots (the host
+ * waits only when a slot's previous copy has not executed yet, i.e. when it
+ * is more than 4 re-bound runs ahead of the device).  The run waits for the
+ * plan's previous work on whatever stream that was enqueued, since it reuses
+ * the workspace.  The bound frames and outputs must stay allocated until the
+ * run completes. */
 int svs_tvl1_plan_run(svs_plan* plan, int use_graph, void* stream);
 
-/* Frees the host side of a plan (the workspace belongs to the caller). */
+/* Waits for the plan's last enqueued work, then frees its host side (the
+ * workspace belongs to the caller and is free to reuse on return). */
 int svs_tvl1_plan_destroy(svs_plan* plan);
 
 /* Launch shape of one level: out[0..7] = bx, by, cz, tiles, blocks, ninst, levels, 0. */
@@ -252,7 +259,9 @@ int svs_tvl1_plan_autotune(svs_plan* plan, int reps, void* stream, int* report);
  * table.  Results do not depend on the shape.  No reference counterpart. */
 int svs_tvl1_plan_set_shape(svs_plan* plan, int level, int bx, int by, int cz, void* stream);
 
-/* One frame pair without a graph: TVL13DOpticalFlow.compute(I0, I1, u, p). */
+/* One frame pair without a graph: TVL13DOpticalFlow.compute(I0, I1, u, p).
+ * Builds a plan in `workspace`, runs it and frees it, so it returns after the
+ * solve has completed; the cached-plan calls above are the asynchronous form. */
 int svs_tvl1_solve(const svs_tvl1_params* params, const float* I0, const float* I1, int NZ, int NY, int NX,
                    const float* u_in, const float* p_in, float* u_out, float* p_out, void* workspace,
                    size_t ws_bytes, void* stream);

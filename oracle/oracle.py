@@ -36,6 +36,14 @@ def lib():
     return _lib
 
 
+def use_library(path=None):
+    """Routes every following call to the oracle build at ``path`` (one of the
+    rounding-sensitivity variants that ``make variants`` writes to
+    oracle/_variants/), or back to the default build when ``path`` is None."""
+    global _lib
+    _lib = None if path is None else ctypes.CDLL(path)
+
+
 def _sfx(dtype):
     return "f32" if np.dtype(dtype) == np.float32 else "f64"
 

@@ -14,6 +14,25 @@
 // -ffp-contract=off so that no multiply-add is fused: the order of rounding is
 // the order written here.
 //
+// Sensitivity variants (oracle/sensitivity.py builds them into
+// oracle/_variants/; the default build defines none of these macros).  Each
+// changes one FP32 rounding choice the reference's own build leaves open, so
+// the spread of their solves measures how far the FP32 result depends on it:
+//   SVS_ORACLE_DOT_ORDER=1 / =2   3-term sums of the driver as (a + b) + c or
+//                                 a + (b + c) instead of torch's (a + c) + b
+//   SVS_ORACLE_FMA                with -ffp-contract=fast: multiply-adds inside
+//                                 the operators fused, as nvcc's default
+//                                 --fmad=true may do (setup.py:43); the steps
+//                                 the reference runs as separate torch kernels
+//                                 stay rounded (round_kernel_output)
+//   SVS_ORACLE_LIBM               glibc expf/powf for the anisotropic weight
+//   SVS_ORACLE_WEIGHT_ULP         the weight moved by one ulp at every other
+//                                 voxel (hash-chosen, either direction), an
+//                                 expf/powf pair with a different 1-ulp error
+//   SVS_ORACLE_BLUR_F64           the pre-blur accumulated in double and
+//                                 rounded once (F.conv3d's summation order is
+//                                 unspecified)
+//
 // Parity status per function (checks live in tests/test_oracle_*.py):
 //   boundary maps ............ brute force over i in [-N, 2N)
 //   Catmull-Rom interpolation  constant/linear/quadratic reproduction, identity
@@ -62,10 +81,47 @@ static inline double sqrt_t(double v) { return sqrt(v); }
 // same source the GPU kernels compile, so both round it identically (the
 // functions themselves are checked against glibc and double-precision libm);
 // the FP64 twin uses libm.
+#ifdef SVS_ORACLE_LIBM
+static inline float exp_t(float v) { return expf(v); }
+static inline float pow_t(float a, float b) { return powf(a, b); }
+#else
 static inline float exp_t(float v) { return svs_expf(v); }
-static inline double exp_t(double v) { return exp(v); }
 static inline float pow_t(float a, float b) { return svs_powf(a, b); }
+#endif
+static inline double exp_t(double v) { return exp(v); }
 static inline double pow_t(double a, double b) { return pow(a, b); }
+// The weight as stored (SVS_ORACLE_WEIGHT_ULP: one ulp off at half the voxels).
+static inline float weight_t(float s, size_t i) {
+#ifdef SVS_ORACLE_WEIGHT_ULP
+  uint32_t h = (uint32_t)i * 2654435761u;
+  h ^= h >> 15;
+  if (h & 1u) return nextafterf(s, (h & 2u) ? 2.0f : 0.0f);
+#else
+  (void)i;
+#endif
+  return s;
+}
+static inline double weight_t(double s, size_t) { return s; }
+// A value the reference computes in a torch kernel of its own reaches the
+// next step rounded to FP32.  In the SVS_ORACLE_FMA variant an empty asm
+// with the value as an in/out register operand keeps the compiler from fusing
+// a multiply across it; otherwise this is the identity.
+template <class T> static inline T round_kernel_output(T v) {
+#ifdef SVS_ORACLE_FMA
+  asm("" : "+x"(v));
+#endif
+  return v;
+}
+// torch.sum over a last dimension of 3 on the GPU (see solve_level).
+template <class T> static inline T torch_sum3(T a, T b, T c) {
+#if SVS_ORACLE_DOT_ORDER == 1
+  return (a + b) + c;
+#elif SVS_ORACLE_DOT_ORDER == 2
+  return a + (b + c);
+#else
+  return (a + c) + b;
+#endif
+}
 static inline float fmax_t(float a, float b) { return fmaxf(a, b); }
 static inline double fmax_t(double a, double b) { return fmax(a, b); }
 
@@ -471,7 +527,7 @@ template <class T> static void aniso_tangents(const T* G, size_t N, T alpha, T b
     for (int d = 0; d < 3; ++d) ns += G[3 * i + d] * G[3 * i + d];
     const T nrm = sqrt_t(ns);
     if (nrm > 0) {
-      s[i] = exp_t(-alpha * pow_t(nrm, beta));
+      s[i] = weight_t(exp_t(-alpha * pow_t(nrm, beta)), i);
       for (int d = 0; d < 3; ++d) n[3 * i + d] = G[3 * i + d] / nrm;
       if (n[3 * i + 2] > (T)0.5) {
         t1[3 * i + 0] = n[3 * i + 2]; t1[3 * i + 1] = 0; t1[3 * i + 2] = -n[3 * i + 0];
@@ -557,7 +613,11 @@ template <class T> static void blur(const T* in, int NZ, int NY, int NX, const T
   for (int z = 0; z < NZ; ++z)
     for (int y = 0; y < NY; ++y)
       for (int x = 0; x < NX; ++x) {
+#ifdef SVS_ORACLE_BLUR_F64
+        double acc = 0;
+#else
         T acc = 0;
+#endif
         for (int a = 0; a < K; ++a) {
           const int zz = z + a - P;
           if (zz < 0 || zz >= NZ) continue;
@@ -567,11 +627,15 @@ template <class T> static void blur(const T* in, int NZ, int NY, int NX, const T
             for (int c = 0; c < K; ++c) {
               const int xx = x + c - P;
               if (xx < 0 || xx >= NX) continue;
+#ifdef SVS_ORACLE_BLUR_F64
+              acc = acc + (double)w[((size_t)a * K + b) * K + c] * (double)in[((size_t)zz * NY + yy) * NX + xx];
+#else
               acc = acc + w[((size_t)a * K + b) * K + c] * in[((size_t)zz * NY + yy) * NX + xx];
+#endif
             }
           }
         }
-        out[((size_t)z * NY + y) * NX + x] = acc;
+        out[((size_t)z * NY + y) * NX + x] = (T)acc;
       }
 }
 
@@ -623,10 +687,16 @@ template <class T> static void solve_level(const SolveParams& P, const Mesh<T>& 
   for (int w = 0; w < P.max_warps; ++w) {
     warp(I1, 1, u.data(), m.NZ, m.NY, m.NX, m.hZ, m.hY, m.hX, P.ip, P.bc, I1w.data());
     warp(G.data(), 3, u.data(), m.NZ, m.NY, m.NX, m.hZ, m.hY, m.hX, P.ip, P.bc, Gw.data());
-    // rho_c = I1w - sum(u * Gw, dim=3) - I0; the 3-term sum is (a + b) + c.
+    // rho_c = I1w - sum(u * Gw, dim=3) - I0.  torch's GPU sum over a last
+    // dimension of 3 rounds as (a + c) + b: its reduction gives that dimension
+    // a block width of 2 (the largest power of two <= 3), so one lane adds
+    // elements 0 and 2 and then combines with the lane holding element 1
+    // (aten/src/ATen/native/cuda/Reduce.cuh, set_block_dimension and
+    // block_x_reduce; the same source in the CUDA and ROCm builds).
     for (size_t i = 0; i < N; ++i) {
-      const T a = u[3 * i] * Gw[3 * i], b = u[3 * i + 1] * Gw[3 * i + 1], c = u[3 * i + 2] * Gw[3 * i + 2];
-      const T sm = (a + b) + c;
+      const T a = round_kernel_output(u[3 * i] * Gw[3 * i]), b = round_kernel_output(u[3 * i + 1] * Gw[3 * i + 1]),
+              c = round_kernel_output(u[3 * i + 2] * Gw[3 * i + 2]);
+      const T sm = torch_sum3(a, b, c);
       rho_c[i] = (I1w[i] - sm) - I0[i];
     }
     for (int it = 0; it < P.max_iters; ++it) {
@@ -635,19 +705,21 @@ template <class T> static void solve_level(const SolveParams& P, const Mesh<T>& 
       nabla_fwd(u.data(), 3, m.NZ, m.NY, m.NX, m.hZ, m.hY, m.hX, P.bc, zg.data());
       const T* dz = zg.data();
       if (P.use_aniso) { aniso_apply(zg.data(), s.data(), n.data(), t1.data(), t2.data(), N, Dz.data()); dz = Dz.data(); }
-      for (size_t k = 0; k < 9 * N; ++k) { const T t = sg * dz[k]; dual[k] = p[k] + t; }
+      for (size_t k = 0; k < 9 * N; ++k) { const T t = round_kernel_output(sg * dz[k]); dual[k] = p[k] + t; }
       prox_dual(dual.data(), wtv, N, pn.data());
       if (P.weight_tv == 0.) std::fill(pn.begin(), pn.end(), (T)0);
       p.swap(pn);
       // primal step with rho evaluated at the current u.
       for (size_t i = 0; i < N; ++i) {
-        const T a = u[3 * i] * Gw[3 * i], b = u[3 * i + 1] * Gw[3 * i + 1], c = u[3 * i + 2] * Gw[3 * i + 2];
-        rho[i] = rho_c[i] + ((a + b) + c);
+        const T a = round_kernel_output(u[3 * i] * Gw[3 * i]),
+                b = round_kernel_output(u[3 * i + 1] * Gw[3 * i + 1]),
+                c = round_kernel_output(u[3 * i + 2] * Gw[3 * i + 2]);
+        rho[i] = rho_c[i] + torch_sum3(a, b, c);
       }
       const T* dp = p.data();
       if (P.use_aniso) { aniso_apply(p.data(), s.data(), n.data(), t1.data(), t2.data(), N, Dp.data()); dp = Dp.data(); }
       nabla_bwd(dp, 3, m.NZ, m.NY, m.NX, m.hZ, m.hY, m.hX, P.bc, div.data());
-      for (size_t k = 0; k < 3 * N; ++k) { const T t = ta * div[k]; primal[k] = u[k] - t; }
+      for (size_t k = 0; k < 3 * N; ++k) { const T t = round_kernel_output(ta * div[k]); primal[k] = u[k] - t; }
       prox_primal(primal.data(), ta, lam, rho.data(), Gw.data(), N, un.data());
       u.swap(un);
     }

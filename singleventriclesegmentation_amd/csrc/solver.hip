@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
   float v[4];
   sample<4, Float4Fetch, SVS_LIN_LOOP>(ip, Float4Fetch{d.J, d.NY, d.NX}, d.NZ, d.NY, d.NX, d.hZ, d.hY, d.hX, bc, cz, cy,
                                        cx, v);
-  const float s = (u0 * v[1] + u1 * v[2]) + u2 * v[3];
+  const float s = (u0 * v[1] + u2 * v[3]) + u1 * v[2];  // torch's sum over dim 3: (a + c) + b
   const float rc = (v[0] - s) - d.I0[i];
   d.RG[i] = make_float4(rc, v[1], v[2], v[3]);
 }
@@ -589,7 +589,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
         const float step = tau * div;
         v[c] = um[c] - step;
       }
-      const float sm = (um[0] * g[0] + um[1] * g[1]) + um[2] * g[2];
+      const float sm = (um[0] * g[0] + um[2] * g[2]) + um[1] * g[1];  // (a + c) + b, as torch's sum
       const float rho = rgc[0] + sm;
       prox_primal_point(v, rho, g, factor);
       for (int c = 0; c < 3; ++c) bst(uout, vo, c * F4 + km * P4, v[c]);
@@ -653,6 +653,18 @@ struct svs_plan {
   hipGraphExec_t exec = nullptr;
   hipStream_t capture_stream = nullptr;
   bool tables_dirty = true;
+  // Completion of the plan's last enqueued work (run, sweeps, autotuning):
+  // work enqueued later, on any stream, waits for it, because it reuses the
+  // workspace and the device tables.
+  hipEvent_t last_work = nullptr;
+  // Re-bound descriptor tables travel through a ring of pinned staging slots,
+  // copied in stream order; a slot is rewritten only after the copy that
+  // last read it has executed (its event).
+  static constexpr int kStagingSlots = 4;
+  char* staging = nullptr;               // kStagingSlots * table_bytes, pinned
+  size_t table_bytes = 0;                // all levels' LevelDesc tables
+  hipEvent_t staged[kStagingSlots] = {};
+  int next_slot = 0;
 };
 
 namespace svs {
@@ -977,13 +989,62 @@ static void fill_schedule(svs_plan* pl) {
     }
 }
 
+// Copies the host descriptor tables to the device in stream order without
+// blocking the host: they are packed into the next pinned staging slot (after
+// waiting, if needed, for the copy that last read that slot) and copied from
+// there on `st`.
 static int upload_tables(svs_plan* pl, hipStream_t st) {
+  const int slot = pl->next_slot;
+  pl->next_slot = (slot + 1) % svs_plan::kStagingSlots;
+  SVS_HIP(hipEventSynchronize(pl->staged[slot]));
+  char* stage = pl->staging + (size_t)slot * pl->table_bytes;
+  size_t off = 0;
   for (int s = 0; s < pl->S; ++s) {
     LevelPlan& L = pl->levels[s];
-    SVS_HIP(hipMemcpyAsync(L.dev, L.host.data(), sizeof(LevelDesc) * pl->ninst, hipMemcpyHostToDevice, st));
+    const size_t n = sizeof(LevelDesc) * pl->ninst;
+    std::memcpy(stage + off, L.host.data(), n);
+    SVS_HIP(hipMemcpyAsync(L.dev, stage + off, n, hipMemcpyHostToDevice, st));
+    off += n;
   }
-  SVS_HIP(hipStreamSynchronize(st));
+  SVS_HIP(hipEventRecord(pl->staged[slot], st));
   pl->tables_dirty = false;
+  return 0;
+}
+
+// Orders work enqueued on `st` after the plan's previous work, whichever
+// stream that ran on.
+static int after_last_work(svs_plan* pl, hipStream_t st) {
+  SVS_HIP(hipStreamWaitEvent(st, pl->last_work, 0));
+  return 0;
+}
+
+static int mark_last_work(svs_plan* pl, hipStream_t st) {
+  SVS_HIP(hipEventRecord(pl->last_work, st));
+  return 0;
+}
+
+// Releases the plan's host-side resources once its last work has completed.
+static void free_plan(svs_plan* pl) {
+  if (pl->last_work) (void)hipEventSynchronize(pl->last_work);
+  for (int i = 0; i < svs_plan::kStagingSlots; ++i)
+    if (pl->staged[i]) {
+      (void)hipEventSynchronize(pl->staged[i]);
+      (void)hipEventDestroy(pl->staged[i]);
+    }
+  if (pl->staging) (void)hipHostFree(pl->staging);
+  if (pl->last_work) (void)hipEventDestroy(pl->last_work);
+  if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+  if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  if (pl->capture_stream) (void)hipStreamDestroy(pl->capture_stream);
+  delete pl;
+}
+
+static int create_sync_objects(svs_plan* pl) {
+  pl->table_bytes = sizeof(LevelDesc) * (size_t)pl->ninst * pl->S;
+  SVS_HIP(hipHostMalloc((void**)&pl->staging, pl->table_bytes * svs_plan::kStagingSlots, hipHostMallocDefault));
+  SVS_HIP(hipEventCreateWithFlags(&pl->last_work, hipEventDisableTiming));
+  for (int i = 0; i < svs_plan::kStagingSlots; ++i)
+    SVS_HIP(hipEventCreateWithFlags(&pl->staged[i], hipEventDisableTiming));
   return 0;
 }
 
@@ -1213,8 +1274,10 @@ int svs_tvl1_plan_create(const svs_tvl1_params* P, int ninst, const int* dims, v
   }
   fill_schedule(pl);
   if (int rc = allow_large_lds(pl)) { delete pl; return rc; }
+  if (int rc = create_sync_objects(pl)) { free_plan(pl); return rc; }
   hipStream_t st = (hipStream_t)stream;
-  if (int rc = upload_static(pl, st)) { delete pl; return rc; }
+  if (int rc = upload_static(pl, st)) { free_plan(pl); return rc; }
+  if (int rc = mark_last_work(pl, st)) { free_plan(pl); return rc; }
   *out = pl;
   return 0;
 }
@@ -1244,9 +1307,13 @@ int svs_tvl1_plan_run(svs_plan* pl, int use_graph, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   for (int k = 0; k < pl->ninst; ++k)
     SVS_REQUIRE(pl->levels[0].host[k].src0, "instance %d has no bound input", k);
+  if (int rc = after_last_work(pl, st)) return rc;
   if (pl->tables_dirty)
     if (int rc = upload_tables(pl, st)) return rc;
-  if (!use_graph) return enqueue_solve(pl, st);
+  if (!use_graph) {
+    if (int rc = enqueue_solve(pl, st)) return rc;
+    return mark_last_work(pl, st);
+  }
   if (!pl->exec) {
     if (!pl->capture_stream) SVS_HIP(hipStreamCreateWithFlags(&pl->capture_stream, hipStreamNonBlocking));
     SVS_HIP(hipStreamBeginCapture(pl->capture_stream, hipStreamCaptureModeThreadLocal));
@@ -1259,15 +1326,12 @@ int svs_tvl1_plan_run(svs_plan* pl, int use_graph, void* stream) {
     SVS_HIP(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
   }
   SVS_HIP(hipGraphLaunch(pl->exec, st));
-  return 0;
+  return mark_last_work(pl, st);
 }
 
 int svs_tvl1_plan_destroy(svs_plan* pl) {
   if (!pl) return 0;
-  if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
-  if (pl->graph) (void)hipGraphDestroy(pl->graph);
-  if (pl->capture_stream) (void)hipStreamDestroy(pl->capture_stream);
-  delete pl;
+  free_plan(pl);
   return 0;
 }
 
@@ -1287,13 +1351,14 @@ int svs_tvl1_plan_sweeps(svs_plan* pl, int level, int count, void* stream) {
   SVS_REQUIRE(!pl->tables_dirty, "run the plan once before timing its sweeps");
   SVS_REQUIRE(pl->P.max_outer_iterations > 0, "no iterations configured");
   hipStream_t st = (hipStream_t)stream;
+  if (int rc = after_last_work(pl, st)) return rc;
   int parity = 0;
   for (int i = 0; i < count; ++i) {
     launch_sweep(pl, pl->levels[level], parity, i % pl->P.max_outer_iterations, st);
     SVS_LAUNCHED();
     parity ^= 1;
   }
-  return 0;
+  return mark_last_work(pl, st);
 }
 
 // Times candidate sweep shapes per level on the plan's own state and keeps
@@ -1305,6 +1370,7 @@ int svs_tvl1_plan_autotune(svs_plan* pl, int reps, void* stream, int* report /* 
   SVS_REQUIRE(!pl->tables_dirty, "run the plan once before autotuning");
   SVS_REQUIRE(pl->P.max_outer_iterations > 0, "no iterations configured");
   hipStream_t st = (hipStream_t)stream;
+  if (int r = after_last_work(pl, st)) return r;
   hipEvent_t e0, e1;
   SVS_HIP(hipEventCreate(&e0));
   SVS_HIP(hipEventCreate(&e1));
@@ -1359,6 +1425,7 @@ int svs_tvl1_plan_autotune(svs_plan* pl, int reps, void* stream, int* report /* 
   (void)hipEventDestroy(e1);
   if (pl->exec) { (void)hipGraphExecDestroy(pl->exec); pl->exec = nullptr; }
   if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  if (!rc) rc = mark_last_work(pl, st);
   return rc;
 }
 
@@ -1369,11 +1436,13 @@ int svs_tvl1_plan_set_shape(svs_plan* pl, int level, int bx, int by, int cz, voi
   SVS_REQUIRE(count_tiles(L, bx, by, cz) <= L.tile_capacity,
               "sweep shape %dx%d, z-chunk %d needs %d tiles; the plan reserved %d", bx, by, cz,
               count_tiles(L, bx, by, cz), L.tile_capacity);
+  hipStream_t st = (hipStream_t)stream;
+  if (int rc = after_last_work(pl, st)) return rc;
   L.bx = bx; L.by = by; L.cz = cz;
-  if (int rc = upload_tiles(pl, level, (hipStream_t)stream)) return rc;
+  if (int rc = upload_tiles(pl, level, st)) return rc;
   if (pl->exec) { (void)hipGraphExecDestroy(pl->exec); pl->exec = nullptr; }
   if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
-  return 0;
+  return mark_last_work(pl, st);
 }
 
 // One frame pair, enqueued directly (no graph): the plain drop-in.

@@ -153,6 +153,8 @@ class Plan:
         self._bound[k] = (I0, I1, u_out, p_out, u_in, p_in)
 
     def run(self, use_graph=True):
+        """Enqueues the solve on the current stream and returns without waiting;
+        it starts after the plan's previous work, whatever stream ran it."""
         with torch.cuda.device(self.device):
             check(_lib.lib().svs_tvl1_plan_run(self._h, 1 if use_graph else 0, stream_ptr()))
 
@@ -184,8 +186,8 @@ class Plan:
 
     def close(self):
         if getattr(self, "_h", None):
-            # A replay of the plan's graph may still be running on the stream.
-            torch.cuda.current_stream(self.device).synchronize()
+            # Waits for the plan's last enqueued work (on whichever stream it
+            # ran), so the workspace can go back to the allocator.
             _lib.lib().svs_tvl1_plan_destroy(self._h)
             self._h = None
 
