@@ -135,6 +135,18 @@ size_t svs_mask_hausdorff_workspace_bytes(int NZ, int NY, int NX);
 int svs_mask_hausdorff2(const float* a, const float* b, int NZ, int NY, int NX, void* workspace, size_t ws_bytes,
                         unsigned long long* out, void* stream);
 
+/* svs_mask_hausdorff2 without a workspace argument and without allocating:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The edge lists live in a fixed 16 MiB device array
+ * that every GPU holds until the process ends, so volumes are limited to
+ * 2^21 voxels (128^3); larger ones are rejected with an error.  out[0..1] as
+ * svs_mask_hausdorff2; edge_counts (device, 2 words, may be NULL) receives
+ * the edge counts of A and B.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (different streams, or threads) race. */
+int svs_mask_hausdorff2_noalloc(const float* a, const float* b, int NZ, int NY, int NX, unsigned long long* out,
+                                unsigned long long* edge_counts, void* stream);
+
 /* ------------------------------------- flow-consistent spatial augmentation */
 
 /* One key of a training sample: src [batch, channels, IZ, IY, IX] ->

@@ -62,6 +62,7 @@ _SIGNATURES = {
     "svs_mask_counts": ([_vp, _vp, _i, _i, _i, _vp, _vp], _i),
     "svs_mask_hausdorff_workspace_bytes": ([_i, _i, _i], ctypes.c_size_t),
     "svs_mask_hausdorff2": ([_vp, _vp, _i, _i, _i, _vp, ctypes.c_size_t, _vp, _vp], _i),
+    "svs_mask_hausdorff2_noalloc": ([_vp, _vp, _i, _i, _i, _vp, _vp, _vp], _i),
     "svs_spatial_augment": ([ctypes.POINTER(AugVolume), _i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _i, _i,
                              _i, _i, _vp], _i),
     "svs_tvl1_workspace_bytes": ([ctypes.POINTER(TVL1Params), _i, ctypes.POINTER(_i)], ctypes.c_size_t),

@@ -15,6 +15,7 @@
 //              The workspace's first two 64-bit words receive the edge counts
 //              of A and B (0 edges: the distance is undefined).
 #include <algorithm>
+#include <mutex>
 
 #include "svs_common.hpp"
 #include "svs_device.hpp"
@@ -119,6 +120,52 @@ __global__ __launch_bounds__(kMetricBlock) void k_directed_hausdorff2(
   if (active && nb > 0) atomicMax(worst, best);
 }
 
+// Scratch of svs_mask_hausdorff2_noalloc: the two edge counts (64 bytes)
+// and two edge lists of up to kNoAllocMaxVoxels entries, 16 MiB + 64 bytes,
+// held by every GPU the library runs on until the process ends.  Host code
+// never passes this symbol to a kernel: its device address comes from
+// hipGetSymbolAddress, once per device (noalloc_scratch).
+constexpr int64_t kNoAllocMaxVoxels = (int64_t)1 << 21;  // 128^3
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[64 + 2 * 4 * kNoAllocMaxVoxels];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the
+// first call there.
+static int noalloc_scratch(void** out) {
+  int dev = -1;
+  SVS_HIP(hipGetDevice(&dev));
+  SVS_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d the library tracks", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    SVS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SVS_REQUIRE(p, "no device address for the scratch array on device %d", dev);
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+// Both directed distances; cnt[0..1] and out[0..1] are zeroed first.
+static int enqueue_hausdorff(const float* a, const float* b, int NZ, int NY, int NX, unsigned long long* cnt,
+                             unsigned int* la, unsigned int* lb, unsigned long long* out, hipStream_t st) {
+  const int64_t n = (int64_t)NZ * NY * NX;
+  k_zero_u64<<<1, 64, 0, st>>>(cnt, 2);
+  k_zero_u64<<<1, 64, 0, st>>>(out, 2);
+  const int grid = grid_for(n, kMetricBlock);
+  k_mask_edges<<<grid, kMetricBlock, 0, st>>>(a, NZ, NY, NX, la, cnt);
+  k_mask_edges<<<grid, kMetricBlock, 0, st>>>(b, NZ, NY, NX, lb, cnt + 1);
+  // Edge counts stay on the device: the launches cover every voxel and
+  // threads past the count return.
+  k_directed_hausdorff2<<<grid, kMetricBlock, 0, st>>>(la, cnt, lb, cnt + 1, NY, NX, out);
+  k_directed_hausdorff2<<<grid, kMetricBlock, 0, st>>>(lb, cnt + 1, la, cnt, NY, NX, out + 1);
+  SVS_LAUNCHED();
+  return SVS_STATUS_OK;
+}
+
 }  // namespace svs
 
 using namespace svs;
@@ -152,22 +199,27 @@ int svs_mask_hausdorff2(const float* a, const float* b, int NZ, int NY, int NX, 
               "mask Hausdorff: workspace of %zu bytes, need %zu", ws_bytes,
               svs_mask_hausdorff_workspace_bytes(NZ, NY, NX));
   SVS_REQUIRE(((uintptr_t)workspace & 7) == 0, "mask Hausdorff: workspace must be 8-byte aligned");
-  hipStream_t st = (hipStream_t)stream;
   const int64_t n = (int64_t)NZ * NY * NX;
   unsigned long long* cnt = (unsigned long long*)workspace;     // [0] |edges A|, [1] |edges B|
   unsigned int* la = (unsigned int*)((char*)workspace + 64);
   unsigned int* lb = la + n;
-  k_zero_u64<<<1, 64, 0, st>>>(cnt, 2);
-  k_zero_u64<<<1, 64, 0, st>>>(out, 2);
-  const int grid = grid_for(n, kMetricBlock);
-  k_mask_edges<<<grid, kMetricBlock, 0, st>>>(a, NZ, NY, NX, la, cnt);
-  k_mask_edges<<<grid, kMetricBlock, 0, st>>>(b, NZ, NY, NX, lb, cnt + 1);
-  // Edge counts stay on the device: the launches cover every voxel and
-  // threads past the count return.
-  k_directed_hausdorff2<<<grid, kMetricBlock, 0, st>>>(la, cnt, lb, cnt + 1, NY, NX, out);
-  k_directed_hausdorff2<<<grid, kMetricBlock, 0, st>>>(lb, cnt + 1, la, cnt, NY, NX, out + 1);
-  SVS_LAUNCHED();
-  return SVS_STATUS_OK;
+  return enqueue_hausdorff(a, b, NZ, NY, NX, cnt, la, lb, out, (hipStream_t)stream);
+}
+
+int svs_mask_hausdorff2_noalloc(const float* a, const float* b, int NZ, int NY, int NX, unsigned long long* out,
+                                unsigned long long* edge_counts, void* stream) {
+  SVS_REQUIRE(a && b && out, "mask Hausdorff (no-alloc): null pointer");
+  SVS_REQUIRE_DIMS(NZ, NY, NX);
+  const int64_t n = (int64_t)NZ * NY * NX;
+  SVS_REQUIRE(n <= kNoAllocMaxVoxels,
+              "mask Hausdorff (no-alloc): %dx%dx%d is %lld voxels, beyond the %lld its fixed scratch holds; "
+              "use svs_mask_hausdorff2 with a workspace", NZ, NY, NX, (long long)n, (long long)kNoAllocMaxVoxels);
+  void* ws = nullptr;
+  if (int rc = noalloc_scratch(&ws)) return rc;
+  unsigned long long* cnt = edge_counts ? edge_counts : (unsigned long long*)ws;
+  unsigned int* la = (unsigned int*)((char*)ws + 64);
+  unsigned int* lb = la + n;
+  return enqueue_hausdorff(a, b, NZ, NY, NX, cnt, la, lb, out, (hipStream_t)stream);
 }
 
 }  // extern "C"
