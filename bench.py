@@ -13,7 +13,11 @@ HBM before timing starts.
 Printed as one JSON line: batched solves/s (the value), the plain per-pair
 drop-in's solves/s, voxel-iterations/s, the HBM roofline fraction of the
 fused sweep kernel (algorithmic bytes over its event-timed average launch
-duration) and the CPU oracle timed on the host cores.
+duration), the CPU oracle timed on the host cores, the reference's
+step-by-step driver over the drop-in operators on the 80^3 pair (speed and
+difference from the fused solve), and a serving leg: Poisson requests of the
+five-size mix through the continuous-batching server, with p50/p99 latency,
+offered load against measured capacity, and the plain per-request drop-in.
 """
 import argparse
 import json
@@ -51,6 +55,12 @@ def parse():
     ap.add_argument("--pairs", type=int, default=96, help="uneven: number of pairs")
     ap.add_argument("--requests", type=int, default=40, help="serve: requests per arrival rate")
     ap.add_argument("--rates", default="5,20,50", help="serve: arrival rates (req/s)")
+    ap.add_argument("--serve-requests", type=int, default=200,
+                    help="sequence workload: requests of the serving leg (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="serving leg: Poisson arrival rate (req/s)")
+    ap.add_argument("--serve-burst", type=int, default=30, help="serving leg: requests of the capacity burst")
+    ap.add_argument("--serve-plain-requests", type=int, default=30,
+                    help="serving leg: requests run through the plain per-request drop-in")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -195,21 +205,6 @@ def main():
         same = bool(torch.equal(u, filtered["u"][args.plain_pairs - 1]))
         log(args, f"plain drop-in: {plain_sps:.2f} solves/s; equal to batched result: {same}")
 
-    # The reference's own driver structure (tvl13d.py:122-255: one operator
-    # call per reference call, ~15 launches per iteration) over the drop-in
-    # operators: the per-pair path a user gets by swapping the import only.
-    op_sps = None
-    if args.operator_pairs > 0:
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for j in range(args.operator_pairs):
-            uo, _ = plain.compute_unfused(seq[j + 1], seq[j], None, None)
-        torch.cuda.synchronize()
-        op_sps = args.operator_pairs / (time.perf_counter() - t0)
-        diff = (uo - u_all[args.operator_pairs - 1]).abs().max().item()
-        log(args, f"operator path: {op_sps:.3f} solves/s; max |u - u_batched| = {diff:.2e} (torch's own "
-                  f"elementwise/sum kernels in the reference driver round in their own order)")
-
     if args.copy_mib > 0:
         src = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=dev).uniform_()
         dst = torch.empty_like(src)
@@ -220,10 +215,32 @@ def main():
     # warped by d, fresh noise): the drop-in's flow against d, and against the
     # FP32 CPU oracle solving the same pair (which is also the timed baseline).
     f0, f1, disp = phantom.frame_pair(shape)
-    u_pair, _ = plain.compute(torch.from_numpy(f1).to(dev), torch.from_numpy(f0).to(dev), None, None)
-    u_pair = u_pair.cpu().numpy()
+    I0_pair, I1_pair = torch.from_numpy(f1).to(dev), torch.from_numpy(f0).to(dev)
+    u_pair_dev, _ = plain.compute(I0_pair, I1_pair, None, None)
+    u_pair = u_pair_dev.cpu().numpy()
     accuracy = {"pair_mean_epe_vs_true_displacement": round(float(np.linalg.norm(u_pair - disp, axis=-1).mean()), 4),
                 "pair_max_displacement": round(float(np.linalg.norm(disp, axis=-1).max()), 3)}
+
+    # The reference's own driver structure (tvl13d.py:122-255: one operator
+    # call per reference call, torch's elementwise and sum kernels between
+    # them, ~15 launches per iteration) over the drop-in operators, on the
+    # same pair: the per-pair path a user gets by swapping the import only.
+    op_sps = None
+    if args.operator_pairs > 0:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.operator_pairs):
+            uo, _ = plain.compute_unfused(I0_pair, I1_pair, None, None)
+        torch.cuda.synchronize()
+        op_sps = args.operator_pairs / (time.perf_counter() - t0)
+        d = (uo.double() - u_pair_dev.double())
+        accuracy["operator_path_max_abs_vs_fused"] = float(d.abs().max().item())
+        accuracy["operator_path_mean_epe_vs_fused"] = float(d.pow(2).sum(-1).sqrt().mean().item())
+        accuracy["operator_path_voxels_above_1e-3"] = int((d.abs().amax(-1) > 1e-3).sum().item())
+        log(args, f"operator path: {op_sps:.3f} solves/s; vs fused: max |du| = "
+                  f"{accuracy['operator_path_max_abs_vs_fused']:.2e}, mean EPE "
+                  f"{accuracy['operator_path_mean_epe_vs_fused']:.2e}, voxels above 1e-3: "
+                  f"{accuracy['operator_path_voxels_above_1e-3']}")
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -288,7 +305,77 @@ def main():
         "cpu_baseline": cpu,
         "accuracy": accuracy,
     }
+    if args.serve_requests > 0:
+        out["served"] = serve_leg(args, dev)
     print(json.dumps(out), flush=True)
+
+
+def serve_leg(args, dev):
+    """Continuous batching under load: Poisson arrivals of 70% single pairs and
+    30% 35-frame forward + backward sequences with mask propagation, sizes
+    from the five-size mix, a 64 GB HBM budget.  Reports the served solves/s
+    and p50/p99 latency at one offered load below capacity, the capacity
+    (all of a burst of requests submitted at once), and the plain per-request
+    drop-in's service times on the first requests of the same stream."""
+    import numpy as np
+    import torch
+    from singleventriclesegmentation_amd import TVL13DOpticalFlow, phantom
+    from singleventriclesegmentation_amd.serve import FlowServer, make_pool, poisson_requests, run_plain, run_service
+    from singleventriclesegmentation_amd.solver import SolverConfig
+    cfg = SolverConfig(num_scales=4, max_warps=10, max_outer_iterations=50)
+    sizes = list(phantom.UNEVEN_SIZES)
+    log(args, "serve: building the request pool (35-frame sequences of the five sizes)")
+    pool = make_pool(sizes, args.frames, dev)
+    buckets = (1, 2, 4)
+    server = FlowServer(cfg, device=dev, hbm_budget_bytes=64 << 30, slots=4, pair_buckets=buckets)
+    server.warm({sh: (pool[sh][0][1], pool[sh][0][0]) for sh in sizes}, frames=args.frames, autotune=False)
+    run_service(server, poisson_requests(0, 6, sizes, pool, args.frames, seed=99))
+
+    burst = poisson_requests(0, args.serve_burst, sizes, pool, args.frames, seed=5)
+    _, span = run_service(server, burst)
+    capacity = sum(r.solves for r in burst) / span
+    log(args, f"serve: capacity {capacity:.1f} solves/s ({len(burst)} requests submitted at once, {span:.1f} s)")
+
+    reqs = poisson_requests(args.serve_rate, args.serve_requests, sizes, pool, args.frames, seed=4)
+    lat, span = run_service(server, reqs)
+    solves = sum(r.solves for r in reqs)
+    arrival_span = max(r.arrival for r in reqs)
+    offered = solves / arrival_span
+    def ms(values, q):
+        return round(1e3 * float(np.percentile(values, q)), 1) if len(values) else None
+
+    pairs = [v for v, r in zip(lat, reqs) if r.kind == "pair"]
+    seqs = [v for v, r in zip(lat, reqs) if r.kind != "pair"]
+    served = {"rate_req_per_s": args.serve_rate, "requests": len(reqs), "solves": solves,
+              "served_solves_per_s": round(solves / span, 2), "p50_ms": ms(lat, 50), "p99_ms": ms(lat, 99),
+              "p50_ms_pairs": ms(pairs, 50), "p99_ms_pairs": ms(pairs, 99),
+              "p50_ms_sequences": ms(seqs, 50), "p99_ms_sequences": ms(seqs, 99),
+              "offered_solves_per_s": round(offered, 2), "capacity_solves_per_s": round(capacity, 2),
+              "offered_over_capacity": round(offered / capacity, 3)}
+    server.close()
+    log(args, f"serve: {served}")
+
+    # The plain per-request drop-in on the first requests of the same stream,
+    # with its per-size plans created before timing.  Its service time per
+    # request is reported; at this arrival rate its queue would grow without
+    # bound when its capacity is below the offered load.
+    plain = TVL13DOpticalFlow(4, 10, 50, 1.0, 0.025, 2, (0.5, 0.5, 0.25, 1.0), True, (5.0, 0.5), True, 1.0,
+                              "CUBIC_HERMITESPLINE", "MIRROR", "numDofs", (1, 1, 1), dev)
+    for sh in sizes:
+        plain.compute(pool[sh][0][1], pool[sh][0][0], None, None)
+    torch.cuda.synchronize()
+    first = poisson_requests(0, args.serve_plain_requests, sizes, pool, args.frames, seed=4)  # back to back
+    lat, span = run_plain(plain, first)
+    service = np.diff(np.concatenate([[0.0], np.sort(lat)]))   # requests ran one after another from t = 0
+    plain_sps = sum(r.solves for r in first) / span
+    served["plain_requests"] = len(first)
+    served["plain_solves_per_s"] = round(plain_sps, 2)
+    served["plain_service_p50_ms"] = ms(service, 50)
+    served["plain_service_p99_ms"] = ms(service, 99)
+    served["plain_offered_over_capacity"] = round(offered / plain_sps, 3)
+    served["served_over_plain"] = round(served["served_solves_per_s"] / plain_sps, 3)
+    log(args, f"serve: plain drop-in {plain_sps:.1f} solves/s on {len(first)} requests back to back")
+    return served
 
 
 def _line(args, metric, value, unit, ms_per_step, config, extra, higher=True):
