@@ -374,6 +374,7 @@ def serve_leg(args, dev):
     served["plain_service_p99_ms"] = ms(service, 99)
     served["plain_offered_over_capacity"] = round(offered / plain_sps, 3)
     served["served_over_plain"] = round(served["served_solves_per_s"] / plain_sps, 3)
+    served["capacity_over_plain"] = round(capacity / plain_sps, 3)
     log(args, f"serve: plain drop-in {plain_sps:.1f} solves/s on {len(first)} requests back to back")
     return served
 
