@@ -610,6 +610,7 @@ const char* svs_last_error(void) { return g_last_error; }
 
 int svs_fill(float* out, int64_t n, float value, void* stream) {
   SVS_REQUIRE(n >= 0 && (n == 0 || out), "svs_fill: bad arguments");
+  SVS_REQUIRE_WORKITEMS(n, "svs_fill");
   return launch_fill(out, n, value, (hipStream_t)stream);
 }
 
@@ -641,6 +642,7 @@ int svs_aniso_tangents(const float* G, int64_t N, float alpha, float beta, float
                        void* stream) {
   SVS_REQUIRE(G && s && n && t1 && t2, "AnisotropicNabla3D.computeTangentVecs: null pointer");
   SVS_REQUIRE(N >= 0, "AnisotropicNabla3D.computeTangentVecs: negative size");
+  SVS_REQUIRE_WORKITEMS(N, "AnisotropicNabla3D.computeTangentVecs");
   if (N == 0) return SVS_STATUS_OK;
   k_aniso_tangents<<<grid_for(N, kBlock), kBlock, 0, (hipStream_t)stream>>>(G, N, alpha, beta, s, n, t1, t2);
   SVS_LAUNCHED();
@@ -651,6 +653,7 @@ int svs_aniso_apply(const float* in, const float* s, const float* n, const float
                     float* out, void* stream) {
   SVS_REQUIRE(in && s && n && t1 && t2 && out, "AnisotropicNabla3D.forwardVectorField: null pointer");
   SVS_REQUIRE(N >= 0, "AnisotropicNabla3D.forwardVectorField: negative size");
+  SVS_REQUIRE_WORKITEMS(N, "AnisotropicNabla3D.forwardVectorField");
   if (N == 0) return SVS_STATUS_OK;
   k_aniso_apply<<<grid_for(N, kBlock), kBlock, 0, (hipStream_t)stream>>>(in, s, n, t1, t2, N, out);
   SVS_LAUNCHED();
@@ -661,6 +664,7 @@ int svs_prox_primal(const float* primal, float tau, float lambda, const float* r
                     float* out, void* stream) {
   SVS_REQUIRE(primal && rho && g && out, "TVL1OF3D_proxPrimal: null pointer");
   SVS_REQUIRE(N >= 0, "TVL1OF3D_proxPrimal: negative size");
+  SVS_REQUIRE_WORKITEMS(N, "TVL1OF3D_proxPrimal");
   if (N == 0) return SVS_STATUS_OK;
   const float factor = tau * lambda;  // opticalFlowOps.cu:442
   k_prox_primal<<<grid_for(N, kBlock), kBlock, 0, (hipStream_t)stream>>>(primal, factor, rho, g, N, out);
@@ -672,6 +676,7 @@ int svs_prox_dual(const float* dual, float sigma, float wtv, int64_t N, float* o
   (void)sigma;  // the reference passes sigma but its kernel never reads it (opticalFlowOps.cu:475-521)
   SVS_REQUIRE(dual && out, "TVL1OF3D_proxDual: null pointer");
   SVS_REQUIRE(N >= 0, "TVL1OF3D_proxDual: negative size");
+  SVS_REQUIRE_WORKITEMS(N, "TVL1OF3D_proxDual");
   if (N == 0) return SVS_STATUS_OK;
   k_prox_dual<<<grid_for(N, kBlock), kBlock, 0, (hipStream_t)stream>>>(dual, wtv, N, out);
   SVS_LAUNCHED();
@@ -698,6 +703,7 @@ int svs_warp3d_cnn(const float* u, const float* phi, int B, int C, int NZ, int N
   SVS_REQUIRE_DIMS(NZ, NY, NX);
   SVS_REQUIRE(valid_bc(bdry) && valid_ip(interp), "WarpingCNN3D.forward: invalid interpolation/boundary type");
   const int64_t n = (int64_t)B * NZ * NY * NX;
+  SVS_REQUIRE_WORKITEMS(n, "WarpingCNN3D.forward (batch x voxels)");
   k_warp_cnn<<<grid_for(n, kBlock), kBlock, 0, (hipStream_t)stream>>>(u, phi, B, C, NZ, NY, NX, hZ, hY, hX, interp,
                                                                       bdry, out);
   SVS_LAUNCHED();
@@ -758,7 +764,7 @@ int svs_median3d(const float* in, int B, int NZ, int NY, int NX, int C, int ks, 
               kMedianMaxK, ks);
   SVS_REQUIRE_DIMS(NZ, NY, NX);
   const int64_t total = (int64_t)B * NZ * NY * NX * C;
-  SVS_REQUIRE(total < ((int64_t)1 << 38), "apply_median_filter: too many elements");
+  SVS_REQUIRE_WORKITEMS(total, "apply_median_filter (batch x voxels x components)");
   if (ks == 3 && total < ((int64_t)1 << 31))
     k_median3<int32_t><<<grid_for(total, kBlock), kBlock, 0, (hipStream_t)stream>>>(in, NZ, NY, NX, C, total, out);
   else if (ks == 3)

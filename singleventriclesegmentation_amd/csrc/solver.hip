@@ -942,6 +942,13 @@ static int validate_params(const svs_tvl1_params* P) {
 
 static int validate_dims(const svs_tvl1_params* P, int ninst, const int* dims) {
   SVS_REQUIRE(ninst >= 1 && dims, "need at least one instance");
+  // The per-voxel kernels of a plan launch one work-item per voxel of every
+  // instance (whole work-groups per instance) at the finest level.
+  int64_t items = 0;
+  for (int k = 0; k < ninst; ++k)
+    if (dims[3 * k] > 0 && dims[3 * k + 1] > 0 && dims[3 * k + 2] > 0)
+      items += ((int64_t)dims[3 * k] * dims[3 * k + 1] * dims[3 * k + 2] + kBlock - 1) / kBlock * kBlock;
+  SVS_REQUIRE_WORKITEMS(items, "plan (voxels of all instances at the finest level)");
   for (int k = 0; k < ninst; ++k) {
     int nz = dims[3 * k], ny = dims[3 * k + 1], nx = dims[3 * k + 2];
     SVS_REQUIRE_DIMS(nz, ny, nx);

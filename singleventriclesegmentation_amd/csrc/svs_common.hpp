@@ -40,6 +40,14 @@ inline bool valid_ip(int ip) { return ip >= 0 && ip <= 2; }
 
 #define SVS_LAUNCHED() SVS_HIP(hipGetLastError())
 
+// A one-dimensional launch covers at most 2^32 - 256 work-items (the grid's
+// work-item count is a 32-bit quantity); kernels that take one work-item per
+// element are bounded by it.
+#define SVS_MAX_WORKITEMS (((int64_t)1 << 32) - 256)
+#define SVS_REQUIRE_WORKITEMS(n, what)                                                                     \
+  SVS_REQUIRE((int64_t)(n) <= SVS_MAX_WORKITEMS, "%s: %lld elements exceed one launch's %lld work-items", \
+              what, (long long)(n), (long long)SVS_MAX_WORKITEMS)
+
 // Dimensions must be positive and the voxel count must fit the int32 index
 // space the reference uses (PackedTensorAccessor32).
 #define SVS_REQUIRE_DIMS(NZ, NY, NX)                                                                  \

@@ -1,9 +1,11 @@
 """Mask-propagation training of svs/models/lit_unet.py on the GPU path.
 
-    PropagationLoss     svs/modules/loss.py:8-64
-    PropagationModule   svs/models/lit_unet.py:13-194 (forward, propagation_step,
-                        forward_and_loss, optimiser and schedule, Dice and
-                        Hausdorff per direction)
+    PropagationLoss     svs/modules/loss.py:8-64 (the same objective, written
+                        over the two chains aligned position by position)
+    PropagationModule   svs/models/lit_unet.py:13-194 (both propagation
+                        chains, advanced together with one batched warp and
+                        one U-Net call per step; forward_and_loss, optimiser
+                        and schedule, Dice and Hausdorff per direction)
     collate_fn          svs/modules/datasets.py:352-409 (NNDataset.collate_fn)
     fit                 scripts/train.py:12-39 with the Trainer behaviour it
                         asks for: a sanity pass over the whole validation set,
@@ -45,37 +47,67 @@ FWD_TS_KEY, BWD_TS_KEY, OFFSET_KEY = "fwd_times", "bwd_times", "offset"
 TOTAL_LOSS_KEY, SUPERVISED_LOSS_KEY, UNSUPERVISED_LOSS_KEY, PENALIZATION_LOSS_KEY = "t", "s", "u", "p"
 
 
+def _align_backward(bwd: torch.Tensor, offsets: torch.Tensor) -> torch.Tensor:
+    """The backward chain re-indexed onto the forward chain's positions.
+
+    A sample whose flows were padded by o steps holds its L = n - o real
+    frames at positions 0..L-1 of the forward chain (the padding steps come
+    after them) and at o..n-1 of the backward chain (the padding steps come
+    first, since the backward chain is stored end-first).  Position j of the
+    result is position min(j + o, n - 1) of ``bwd``."""
+    bs, n = bwd.shape[:2]
+    pos = torch.arange(n, device=bwd.device)
+    src = torch.clamp(pos[None, :] + offsets[:, None], max=n - 1)
+    return torch.gather(bwd, 1, src.view(bs, n, *([1] * (bwd.dim() - 2))).expand_as(bwd))
+
+
+def _volume_sums(x: torch.Tensor) -> torch.Tensor:
+    """Sum over every axis after the first two: [B, T, ...] -> [B, T]."""
+    return x.flatten(2).sum(2)
+
+
 class PropagationLoss(nn.Module):
-    """loss.py:8-64: supervised MSE at the two labelled frames, the
-    forward/backward agreement on the interior frames and a penalty on the
-    raw U-Net outputs, the last two divided by each sample's chain length."""
+    """The propagation objective of svs/modules/loss.py:8-64.
+
+    With the backward chain aligned onto the forward one (_align_backward),
+    every term is a per-position volume sum over a sample's L real frames:
+
+    * supervised: the squared error at the first position (the backward
+      chain's estimate of the given initial mask) plus the one at position
+      L - 1 (the forward chain's estimate of the given final mask), summed
+      over the batch and divided by its size;
+    * unsupervised: the squared disagreement of the two chains at the
+      interior positions 1..L-2, divided by L per sample, times
+      lambda_u / batch size;
+    * penalty: the squared raw U-Net outputs of the L - 1 real steps of both
+      chains, divided by L per sample, times gamma_p / batch size.
+    """
 
     def __init__(self, lambda_u: float, gamma_p: float):
         super().__init__()
         self.lambda_u = lambda_u
         self.gamma_p = gamma_p
-        self.mse_fn = nn.MSELoss(reduction="sum")
 
     def forward(self, y: Tuple[torch.Tensor, ...], offsets: torch.Tensor) -> Dict[str, torch.Tensor]:
-        mts, mtts, mhs, mhhs = y
-        bs = mts.shape[0]
-        bi = torch.arange(bs)
-        sup_1 = self.mse_fn(mtts[bi, offsets[bi]], mts[bi, 0])
-        sup_2 = self.mse_fn(mts[bi, -offsets[bi] - 1], mtts[bi, -1])
-        kl = mts.shape[1] - offsets
-        n_tilde, n_hat = mts.shape[1], mhs.shape[1]
-        unsup = torch.as_tensor(0.0, device=mts.device)
-        pen = torch.as_tensor(0.0, device=mts.device)
-        for b in range(bs):
-            o = int(offsets[b])
-            unsup = unsup + self.mse_fn(mts[b, 1:n_tilde - o - 1], mtts[b, 1 + o:-1]) / kl[b]
-            mh, mhh = mhs[b, 0:n_hat - o], mhhs[b, 0:n_hat - o]
-            pen = pen + (mh.pow(2).sum() + mhh.pow(2).sum()) / kl[b]
-        sup = (sup_1 + sup_2) / bs
-        unsup = unsup * (self.lambda_u / bs)
-        pen = pen * (self.gamma_p / bs)
-        return {TOTAL_LOSS_KEY: sup + unsup + pen, SUPERVISED_LOSS_KEY: sup, UNSUPERVISED_LOSS_KEY: unsup,
-                PENALIZATION_LOSS_KEY: pen}
+        fwd, bwd, raw_fwd, raw_bwd = y
+        bs, n = fwd.shape[:2]
+        dev = fwd.device
+        off = offsets.to(device=dev, dtype=torch.long)
+        length = n - off
+        err = _volume_sums((fwd - _align_backward(bwd, off)).pow(2))          # [bs, n]
+        last = (length - 1)[:, None]
+        # Both ends are counted, also when they coincide (a single real frame).
+        supervised = (err[:, 0] + err.gather(1, last)[:, 0]).sum() / bs
+        pos = torch.arange(n, device=dev)[None, :]
+        interior = (pos > 0) & (pos < last)
+        agreement = torch.where(interior, err, torch.zeros_like(err)).sum(1) / length
+        unsupervised = agreement.sum() * (self.lambda_u / bs)
+        steps = raw_fwd.shape[1]
+        real = torch.arange(steps, device=dev)[None, :] < (steps - off)[:, None]
+        raw = _volume_sums(raw_fwd.pow(2)) + _volume_sums(raw_bwd.pow(2))       # [bs, steps]
+        penalty = (torch.where(real, raw, torch.zeros_like(raw)).sum(1) / length).sum() * (self.gamma_p / bs)
+        return {TOTAL_LOSS_KEY: supervised + unsupervised + penalty, SUPERVISED_LOSS_KEY: supervised,
+                UNSUPERVISED_LOSS_KEY: unsupervised, PENALIZATION_LOSS_KEY: penalty}
 
 
 def collate_fn(batch: List[Dict[str, Any]]) -> Dict[str, Any]:
@@ -168,37 +200,46 @@ class PropagationModule(nn.Module):
         self.metrics = {stage: [_MaskMetric(), _MaskMetric()] for stage in ("trn", "val")}   # [b, f]
 
     def configure_optimizers(self, lr: Optional[float] = None):
-        """lit_unet.py:60-64: Adam and a cosine schedule down to lr / 100."""
-        lr = lr or self.lr
-        opt = torch.optim.Adam(self.parameters(), lr=lr, betas=self.betas, weight_decay=self.weight_decay)
-        sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=self.epochs, eta_min=lr / 100)
-        return opt, sched
+        """Adam with the module's betas and weight decay, and a cosine
+        schedule from ``lr`` down to lr / 100 over the epochs (stepped once
+        per epoch), as lit_unet.py:60-64 configures them."""
+        base = lr or self.lr
+        optimiser = torch.optim.Adam(self.parameters(), lr=base, betas=self.betas, weight_decay=self.weight_decay)
+        schedule = torch.optim.lr_scheduler.CosineAnnealingLR(optimiser, T_max=self.epochs, eta_min=base / 100)
+        return optimiser, schedule
 
     def forward(self, batch: Dict[str, Any]) -> Tuple[torch.Tensor, ...]:
-        """lit_unet.py:66-92: both propagation chains over the padded length."""
-        img, mi, mf = batch[IMAGE_KEY], batch[MI_KEY], batch[MF_KEY]
-        fflow, bflow = batch[FWD_FLOW_KEY], batch[BWD_FLOW_KEY]
-        ftimes, btimes = batch[FWD_TS_KEY], batch[BWD_TS_KEY]
-        bs, ts, nz, ny, nx, _ = fflow.shape
-        bi = torch.arange(bs)
-        mts = torch.empty((bs, ts + 1, 1, nz, ny, nx), dtype=mi.dtype, device=mi.device)
-        mts[:, 0] = mi
-        mtts = torch.empty_like(mts)
-        mtts[:, -1] = mf
-        mhs = torch.empty((bs, ts, 1, nz, ny, nx), dtype=mi.dtype, device=mi.device)
-        mhhs = torch.empty_like(mhs)
-        for i in range(ts):
-            mts[:, i + 1], mhs[:, i] = self.propagation_step(mts[:, i], img, fflow[:, i], ftimes, i, bi)
-            mtts[:, ts - i - 1], mhhs[:, ts - i - 1] = self.propagation_step(mtts[:, ts - i], img, bflow[:, i],
-                                                                              btimes, i, bi)
-        return mts, mtts, mhs, mhhs
+        """Both propagation chains over the padded length (lit_unet.py:66-120).
 
-    def propagation_step(self, mask, img, flow, times, i, bi):
-        """lit_unet.py:104-120: warp the mask, stack it with the target frame
-        and let the U-Net refine it."""
-        warped = self.warp(mask, flow)
-        x = torch.cat((img[bi, times[bi, i + 1]], warped), dim=1)
-        return self.model(x)
+        The chains advance together: step i warps the forward chain's
+        latest mask by forward flow i and the backward chain's latest mask
+        by backward flow i as one batch of 2 x batch volumes (one warp
+        launch), stacks each with the frame its chain reaches next, and
+        refines all of them in one U-Net call.  The U-Net normalises per
+        sample (instance norm), so batching the directions changes no
+        sample's arithmetic.  Returns (forward chain, backward chain, raw
+        forward outputs, raw backward outputs): chains [B, T+1, 1, Z, Y, X]
+        indexed by frame from the initial to the final mask, raw outputs
+        [B, T, 1, Z, Y, X] indexed the same way (the backward chain's step
+        that produced frame t is stored at t)."""
+        img = batch[IMAGE_KEY]
+        flows = torch.cat((batch[FWD_FLOW_KEY], batch[BWD_FLOW_KEY]))        # [2B, T, Z, Y, X, 3]
+        times = torch.cat((batch[FWD_TS_KEY], batch[BWD_TS_KEY]))            # [2B, T+1]
+        bs, steps = batch[FWD_FLOW_KEY].shape[:2]
+        sample = torch.arange(2 * bs, device=times.device) % bs
+        current = torch.cat((batch[MI_KEY], batch[MF_KEY]))                  # latest mask of each chain
+        refined, raw = [current], []
+        for i in range(steps):
+            target = img[sample, times[:, i + 1]]
+            nxt, out = self.model(torch.cat((target, self.warp(current, flows[:, i])), dim=1))
+            refined.append(nxt)
+            raw.append(out)
+            current = nxt
+        fwd = torch.stack([m[:bs] for m in refined], dim=1)
+        bwd = torch.stack([m[bs:] for m in reversed(refined)], dim=1)
+        raw_fwd = torch.stack([m[:bs] for m in raw], dim=1)
+        raw_bwd = torch.stack([m[bs:] for m in reversed(raw)], dim=1)
+        return fwd, bwd, raw_fwd, raw_bwd
 
     def forward_and_loss(self, batch):
         y = self(batch)
@@ -206,14 +247,18 @@ class PropagationModule(nn.Module):
 
     @torch.no_grad()
     def update_metrics(self, stage: str, y, offsets: torch.Tensor):
-        """lit_unet.py:178-194: the labelled end of each chain against the
-        other chain's prediction there, thresholded at 0.5."""
-        mts, mtts = y[0], y[1]
-        bi = torch.arange(mts.shape[0])
-        mi, mitt = mts[bi, 0], mtts[bi, offsets[bi]]
-        mf, mft = mtts[bi, -1], mts[bi, -offsets[bi] - 1]
-        for k, (target, pred) in enumerate(((mi, mitt), (mf, mft))):
-            self.metrics[stage][k].update(torch.where(pred > 0.5, 1.0, 0.0), target)
+        """Scores each chain's estimate of the other chain's given mask
+        (lit_unet.py:178-194): the backward chain at the initial frame
+        against the initial mask ("b"), the forward chain at the final frame
+        against the final mask ("f"), thresholded at 0.5."""
+        fwd, bwd = y[0], y[1]
+        n = fwd.shape[1]
+        off = offsets.to(device=fwd.device, dtype=torch.long)
+        rows = torch.arange(fwd.shape[0], device=fwd.device)
+        estimates = (bwd[rows, off], fwd[rows, n - 1 - off])
+        given = (fwd[:, 0], bwd[:, n - 1])
+        for k in range(2):
+            self.metrics[stage][k].update(torch.where(estimates[k] > 0.5, 1.0, 0.0), given[k])
 
     def compute_metrics(self, stage: str) -> Dict[str, float]:
         """compute_and_log_mean_metrics (lit_unet.py:152-170)."""

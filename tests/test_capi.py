@@ -57,3 +57,20 @@ def test_plan_rejects_instances_beyond_32bit_offsets():
     assert b"exceeds" in _lib.lib().svs_last_error()
     odd = (ctypes.c_int * 6)(16, 16, 16, 0, 16, 16)  # a zero-size second instance
     assert _lib.lib().svs_tvl1_workspace_bytes(ctypes.byref(p), 2, odd) == 0
+
+
+def test_one_dimensional_launch_limits_are_rejected_up_front():
+    """Element counts beyond a 1-D launch's 2^32 - 256 work-items are refused
+    by argument validation, before any HIP call (so this runs without a
+    GPU; the pointers are never dereferenced)."""
+    from singleventriclesegmentation_amd import _lib
+    lib = _lib.lib()
+    fake, fake_out = 256, 512
+    # median filter: 2 x 1024^3 x 3 elements (the volume itself is legal)
+    assert lib.svs_median3d(fake, 2, 1024, 1024, 1024 - 1, 3, 3, fake_out, None) != 0
+    assert "work-items" in lib.svs_last_error().decode()
+    # CNN warp: 3 x 1290^3 work-items
+    assert lib.svs_warp3d_cnn(fake, fake, 3, 1, 1290, 1290, 1290, 1.0, 1.0, 1.0, 2, 1, fake_out, None) != 0
+    assert "work-items" in lib.svs_last_error().decode()
+    assert lib.svs_prox_dual(fake, 0.5, 0.025, (1 << 32), fake_out, None) != 0
+    assert "work-items" in lib.svs_last_error().decode()

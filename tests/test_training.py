@@ -3,6 +3,7 @@ svs/modules/datasets.py collate_fn): the collate rules on hand-made samples
 and the loss against a float64 loop restatement of loss.py:21-64 plus
 closed forms."""
 import numpy as np
+import pytest
 import torch
 
 from singleventriclesegmentation_amd import training as TR
@@ -53,14 +54,17 @@ def loss_restated(mts, mtts, mhs, mhhs, offsets, lam, gam):
     return {"t": sup + unsup + pen, "s": sup, "u": unsup, "p": pen}
 
 
-def test_loss_matches_loop_restatement():
+@pytest.mark.parametrize("offsets", [[2, 0, 1], [5, 0, 3], [0, 0, 0], [4, 1, 5]])
+def test_loss_matches_loop_restatement(offsets):
+    """Including samples with a single real frame (offset = T), whose two
+    supervised ends coincide and are both counted."""
     g = torch.Generator().manual_seed(4)
     bs, ts, shape = 3, 5, (4, 5, 6)
     mts = torch.rand(bs, ts + 1, 1, *shape, generator=g)
     mtts = torch.rand(bs, ts + 1, 1, *shape, generator=g)
     mhs = torch.randn(bs, ts, 1, *shape, generator=g)
     mhhs = torch.randn(bs, ts, 1, *shape, generator=g)
-    offsets = torch.tensor([2, 0, 1])
+    offsets = torch.tensor(offsets)
     got = TR.PropagationLoss(1.0, 0.1)((mts, mtts, mhs, mhhs), offsets)
     want = loss_restated(mts, mtts, mhs, mhhs, offsets, 1.0, 0.1)
     for k in "tsup":
