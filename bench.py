@@ -681,7 +681,7 @@ def bench_train(args, dev):
           round(args.steps / el, 3), "steps/s", round(1e3 * el / args.steps, 2),
           {"workload": f"batch 4 x {shape[0]}^3, chains of {ts} flows (ED->ES 8..11, padded), U-Net 4 levels / "
                        f"8 channels, LINEAR warp, rotation + 3 flips per sample on the GPU",
-           "unet_calls_per_step": 2 * ts},
+           "unet_calls_per_step": ts, "unet_batch": 2 * int(batch["fwd_flow"].shape[0])},
           {"train_only_steps_per_s": round(args.steps / el_train, 3),
            "loss": round(float(loss["t"].detach()), 4), "samples_per_s": round(4 * args.steps / el, 2)})
 
