@@ -246,7 +246,9 @@ int svs_tvl1_plan_run(svs_plan* plan, int use_graph, void* stream);
  * workspace belongs to the caller and is free to reuse on return). */
 int svs_tvl1_plan_destroy(svs_plan* plan);
 
-/* Launch shape of one level: out[0..7] = bx, by, cz, tiles, blocks, ninst, levels, 0. */
+/* Launch shape of one level: out[0..7] = bx, by, cz, tiles, blocks, ninst,
+ * levels, and the resident sweep's voxels per thread (0: the level runs the
+ * marching sweep; see svs_tvl1_plan_set_resident). */
 int svs_tvl1_plan_info(const svs_plan* plan, int level, int* out);
 
 /* Enqueues `count` fused primal-dual sweeps of pyramid level `level` on the
@@ -270,6 +272,14 @@ int svs_tvl1_plan_autotune(svs_plan* plan, int reps, void* stream, int* report);
  * Fails if a resulting tile exceeds the kernel's limits or the plan's tile
  * table.  Results do not depend on the shape.  No reference counterpart. */
 int svs_tvl1_plan_set_shape(svs_plan* plan, int level, int bx, int by, int cz, void* stream);
+
+/* Chooses how a level runs its primal-dual iterations and drops the captured
+ * graph: mode 1 runs each warp's iterations in one resident launch (one
+ * work-group per instance holding the state on chip; only for levels whose
+ * instances have at most 2048 voxels), 0 launches the marching sweep once per
+ * iteration, -1 restores the automatic choice (resident wherever the level
+ * qualifies).  Results do not depend on the mode.  No reference counterpart. */
+int svs_tvl1_plan_set_resident(svs_plan* plan, int level, int mode, void* stream);
 
 /* One frame pair without a graph: TVL13DOpticalFlow.compute(I0, I1, u, p).
  * Builds a plan in `workspace`, runs it and frees it, so it returns after the

@@ -615,6 +615,180 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   }
 }
 
+// ------------------------------------------- resident multi-iteration sweep
+// All K primal-dual iterations of one warp in a single launch, for pyramid
+// levels whose instances hold at most kResidentPerThread * 1024 voxels: one
+// work-group per instance keeps the instance's whole state on chip for the
+// warp.  A thread owns up to V voxels (v = tid + j * blockDim.x) and holds
+// their u, p, (s, n) and (rho_c, g) in registers; LDS holds the two fields
+// neighbours read: u (for grad u) and r = D p' (for the adjoint), 48 bytes
+// per voxel.  Per iteration:
+//   1. own u -> LDS;                                           barrier
+//   2. dual step from the neighbours' u; own r -> LDS;         barrier
+//   3. primal step from the neighbours' r (registers only).
+// Step 3 reads only r and the next step 1 writes only u; the next step 2
+// writes r after the barrier that every thread reaches after its step 3.
+// Each voxel is updated with the same operations, in the same order, as the
+// marching sweep (k_sweep), so the results are the same bits; the launch
+// replaces K launches whose cost at these sizes is latency.
+// sched holds (sigma, tau, tau * lambda) per iteration; the state is read
+// from buffer `parity` and left in buffer parity ^ (K & 1), where K single
+// sweeps would leave it.  With two voxels per thread, (s, n) and (rho_c, g)
+// are re-read from global memory (cache-resident after the first iteration)
+// instead of held in registers, which keeps the kernel within 128 VGPRs
+// without spills.
+constexpr int kResidentPerThread = 2;
+
+template <int BC, bool ANISO, bool UNIT_H, int V>
+__global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __restrict__ descs,
+                                                         const float* __restrict__ sched, int first, int K,
+                                                         int sched_len, int parity, float wtv, int wtv_zero) {
+  extern __shared__ float lds[];
+  const LevelDesc& d = descs[blockIdx.x];
+  const int NZ = d.NZ, NY = d.NY, NX = d.NX;
+  const int N = (int)d.N;
+  const int PL = NY * NX;
+  const float hZ = UNIT_H ? 1.0f : d.hZ, hY = UNIT_H ? 1.0f : d.hY, hX = UNIT_H ? 1.0f : d.hX;
+  float* U = lds;          // [3][N]
+  float* R = lds + 3 * N;  // [9][N]: r[3c + e] = (D p')[c][e]
+  const float* uin = d.u[parity];
+  const float* pin = d.p[parity];
+
+  constexpr bool KEEP = V == 1;  // (s, n) and (rho_c, g) held in registers
+  float u[V][3], p[V][9], sn[KEEP ? V : 1][4], rg[KEEP ? V : 1][4];
+  int vz[V], vy[V], vx[V];
+  bool own[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    const int v = (int)threadIdx.x + j * (int)blockDim.x;
+    own[j] = v < N;
+    const int vv = own[j] ? v : 0;
+    vx[j] = vv % NX;
+    vy[j] = (vv / NX) % NY;
+    vz[j] = vv / PL;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u[j][c] = own[j] ? uin[(int64_t)c * N + vv] : 0.0f;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) p[j][e] = own[j] ? pin[(int64_t)e * N + vv] : 0.0f;
+    if (KEEP) {
+      const float4 a = own[j] && ANISO ? d.SN[vv] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const float4 b = own[j] ? d.RG[vv] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      sn[0][0] = a.x; sn[0][1] = a.y; sn[0][2] = a.z; sn[0][3] = a.w;
+      rg[0][0] = b.x; rg[0][1] = b.y; rg[0][2] = b.z; rg[0][3] = b.w;
+    }
+  }
+
+  for (int it = 0; it < K; ++it) {
+    const int n = (first + it) % sched_len;
+    const float sigma = sched[3 * n], tau = sched[3 * n + 1], factor = sched[3 * n + 2];
+#pragma unroll
+    for (int j = 0; j < V; ++j)
+      if (own[j]) {
+        const int v = (int)threadIdx.x + j * (int)blockDim.x;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) U[c * N + v] = u[j][c];
+      }
+    __syncthreads();
+    // ---- dual step
+#pragma unroll
+    for (int j = 0; j < V; ++j)
+      if (own[j]) {
+        const int v = (int)threadIdx.x + j * (int)blockDim.x;
+        const int x = vx[j], y = vy[j], z = vz[j];
+        // Neighbour indices clamped into the volume: a clamped value only
+        // reaches derivative branches the boundary rule does not select.
+        const int xm = x > 0 ? v - 1 : v, xp = x < NX - 1 ? v + 1 : v;
+        const int ym = y > 0 ? v - NX : v, yp = y < NY - 1 ? v + NX : v;
+        const int zm = z > 0 ? v - PL : v, zp = z < NZ - 1 ? v + PL : v;
+        float s = 0.0f, nn[3] = {0.0f, 0.0f, 0.0f}, t1[3] = {0.0f, 0.0f, 0.0f};
+        if (ANISO) {
+          if (KEEP) {
+            s = sn[0][0]; nn[0] = sn[0][1]; nn[1] = sn[0][2]; nn[2] = sn[0][3];
+          } else {
+            const float4 a = d.SN[v];
+            s = a.x; nn[0] = a.y; nn[1] = a.z; nn[2] = a.w;
+          }
+          tangent1(nn, t1);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float* Uc = U + c * N;
+          const float b0 = Uc[v];
+          float q[3];
+          q[0] = cd_fwd(Uc[xm], b0, Uc[xp], x, NX, hX, BC);
+          q[1] = cd_fwd(Uc[ym], b0, Uc[yp], y, NY, hY, BC);
+          q[2] = cd_fwd(Uc[zm], b0, Uc[zp], z, NZ, hZ, BC);
+          float dq[3];
+          if (ANISO) aniso_row(s, nn, t1, q, dq);
+          else { dq[0] = q[0]; dq[1] = q[1]; dq[2] = q[2]; }
+          float pv[3];
+#pragma unroll
+          for (int e = 0; e < 3; ++e) {
+            const float inc = sigma * dq[e];
+            pv[e] = p[j][3 * c + e] + inc;
+          }
+          prox_dual_row(pv, wtv);
+          if (wtv_zero) { pv[0] = 0.0f; pv[1] = 0.0f; pv[2] = 0.0f; }
+#pragma unroll
+          for (int e = 0; e < 3; ++e) p[j][3 * c + e] = pv[e];
+          float rr[3];
+          if (ANISO) aniso_row(s, nn, t1, pv, rr);
+          else { rr[0] = pv[0]; rr[1] = pv[1]; rr[2] = pv[2]; }
+#pragma unroll
+          for (int e = 0; e < 3; ++e) R[(3 * c + e) * N + v] = rr[e];
+        }
+      }
+    __syncthreads();
+    // ---- primal step
+#pragma unroll
+    for (int j = 0; j < V; ++j)
+      if (own[j]) {
+        const int v = (int)threadIdx.x + j * (int)blockDim.x;
+        const int x = vx[j], y = vy[j], z = vz[j];
+        const int xm = x > 0 ? v - 1 : v, xp = x < NX - 1 ? v + 1 : v;
+        const int ym = y > 0 ? v - NX : v, yp = y < NY - 1 ? v + NX : v;
+        const int zm = z > 0 ? v - PL : v, zp = z < NZ - 1 ? v + PL : v;
+        float rgv[4];
+        if (KEEP) {
+          rgv[0] = rg[0][0]; rgv[1] = rg[0][1]; rgv[2] = rg[0][2]; rgv[3] = rg[0][3];
+        } else {
+          const float4 b = d.RG[v];
+          rgv[0] = b.x; rgv[1] = b.y; rgv[2] = b.z; rgv[3] = b.w;
+        }
+        const float g[3] = {rgv[1], rgv[2], rgv[3]};
+        float nv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float* Rx = R + (3 * c) * N;
+          const float* Ry = R + (3 * c + 1) * N;
+          const float* Rz = R + (3 * c + 2) * N;
+          const float ax = cd_adj(Rx[xm], Rx[v], Rx[xp], x, NX, BC);
+          const float ay = cd_adj(Ry[ym], Ry[v], Ry[yp], y, NY, BC);
+          const float az = cd_adj(Rz[zm], Rz[v], Rz[zp], z, NZ, BC);
+          const float div = ax / hX + ay / hY + az / hZ;
+          const float step = tau * div;
+          nv[c] = u[j][c] - step;
+        }
+        const float sm = (u[j][0] * g[0] + u[j][2] * g[2]) + u[j][1] * g[1];  // (a + c) + b, as torch's sum
+        const float rho = rgv[0] + sm;
+        prox_primal_point(nv, rho, g, factor);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[j][c] = nv[c];
+      }
+  }
+  float* uout = d.u[parity ^ (K & 1)];
+  float* pout = d.p[parity ^ (K & 1)];
+#pragma unroll
+  for (int j = 0; j < V; ++j)
+    if (own[j]) {
+      const int v = (int)threadIdx.x + j * (int)blockDim.x;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) uout[(int64_t)c * N + v] = u[j][c];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) pout[(int64_t)e * N + v] = p[j][e];
+    }
+}
+
 // ============================================================ host plan
 struct LevelPlan {
   std::vector<LevelDesc> host;   // per instance
@@ -634,6 +808,12 @@ struct LevelPlan {
   int threads = 0;               // work-group size: the largest tile's bx * by
   size_t lds = 0;                // dynamic LDS of the largest tile
   bool unit_h = false;           // h == 1 on every axis of every instance
+  // Resident sweep (k_sweep_resident): voxels per thread (0: the marching
+  // sweep runs this level), its work-group size and dynamic LDS.
+  int resident = 0;
+  int res_capable = 0;           // voxels per thread the level would need (0: too large)
+  int res_threads = 0;
+  size_t res_lds = 0;
 };
 
 }  // namespace svs
@@ -665,6 +845,7 @@ struct svs_plan {
   size_t table_bytes = 0;                // all levels' LevelDesc tables
   hipEvent_t staged[kStagingSlots] = {};
   int next_slot = 0;
+  float* sched_dev = nullptr;            // (sigma, tau, tau * lambda) per iteration
 };
 
 namespace svs {
@@ -918,6 +1099,19 @@ static int plan_layout(svs_plan* pl, char* base, size_t* total_out) {
     }
     L.nblocks = nb;
     L.unit_h = unit;
+    {
+      int64_t maxN = 0;
+      for (auto& d : L.host) maxN = std::max<int64_t>(maxN, d.N);
+      L.res_capable = 0;
+      for (int v = 1; v <= kResidentPerThread && !L.res_capable; ++v)
+        if (maxN <= (int64_t)1024 * v) L.res_capable = v;
+      if (L.res_capable) {
+        const int per = (int)((maxN + L.res_capable - 1) / L.res_capable);
+        L.res_threads = std::min(1024, (per + 63) / 64 * 64);
+        L.res_lds = sizeof(float) * 12 * (size_t)maxN;
+      }
+      L.resident = L.res_capable;  // automatic: resident whenever the level qualifies
+    }
     L.px_blocks = L.py_blocks = 0;
     if (s + 1 < S)
       for (int k = 0; k < pl->ninst; ++k) {
@@ -926,6 +1120,7 @@ static int plan_layout(svs_plan* pl, char* base, size_t* total_out) {
       }
   }
   if (P.use_blur) pl->blur_dev = (float*)take(sizeof(float) * pl->blur_w.size());
+  pl->sched_dev = (float*)take(sizeof(float) * 3 * (size_t)std::max(1, P.max_outer_iterations));
   *total_out = align_up(off, 256);
   return 0;
 }
@@ -1087,6 +1282,16 @@ static int upload_static(svs_plan* pl, hipStream_t st) {
                            hipMemcpyHostToDevice, st));
     SVS_HIP(hipStreamSynchronize(st));
   }
+  if (pl->P.max_outer_iterations > 0) {
+    std::vector<float> sched(3 * (size_t)pl->P.max_outer_iterations);
+    for (int n = 0; n < pl->P.max_outer_iterations; ++n) {
+      sched[3 * n] = pl->sig[n];
+      sched[3 * n + 1] = pl->tau[n];
+      sched[3 * n + 2] = pl->factor[n];
+    }
+    SVS_HIP(hipMemcpyAsync(pl->sched_dev, sched.data(), sizeof(float) * sched.size(), hipMemcpyHostToDevice, st));
+    SVS_HIP(hipStreamSynchronize(st));
+  }
   return 0;
 }
 
@@ -1161,6 +1366,53 @@ static void launch_sweep(const svs_plan* pl, const LevelPlan& L, int parity, int
   }
 }
 
+template <int BC, bool ANISO, int V>
+static void launch_resident_v(const svs_plan* pl, const LevelPlan& L, int parity, int first, int K,
+                              hipStream_t st) {
+  const svs_tvl1_params& P = pl->P;
+  const float wtv = P.weight_tv;
+  const int wz = P.weight_tv == 0.0f;
+  const int len = P.max_outer_iterations;
+  if (L.unit_h)
+    hipLaunchKernelGGL((k_sweep_resident<BC, ANISO, true, V>), dim3(pl->ninst), dim3(L.res_threads), L.res_lds, st,
+                       L.dev, pl->sched_dev, first, K, len, parity, wtv, wz);
+  else
+    hipLaunchKernelGGL((k_sweep_resident<BC, ANISO, false, V>), dim3(pl->ninst), dim3(L.res_threads), L.res_lds,
+                       st, L.dev, pl->sched_dev, first, K, len, parity, wtv, wz);
+}
+
+template <int BC, bool ANISO>
+static void launch_resident_t(const svs_plan* pl, const LevelPlan& L, int parity, int first, int K,
+                              hipStream_t st) {
+  if (L.resident == 1) launch_resident_v<BC, ANISO, 1>(pl, L, parity, first, K, st);
+  else launch_resident_v<BC, ANISO, 2>(pl, L, parity, first, K, st);
+}
+
+// K iterations of schedule positions first, first + 1, ... (cyclic) in one
+// launch; the state moves from buffer `parity` to parity ^ (K & 1).
+static void launch_resident(const svs_plan* pl, const LevelPlan& L, int parity, int first, int K,
+                            hipStream_t st) {
+  const bool an = pl->P.use_aniso != 0;
+  switch (pl->P.boundary * 2 + (an ? 1 : 0)) {
+    case 0: launch_resident_t<BD_NEAREST, false>(pl, L, parity, first, K, st); break;
+    case 1: launch_resident_t<BD_NEAREST, true>(pl, L, parity, first, K, st); break;
+    case 2: launch_resident_t<BD_MIRROR, false>(pl, L, parity, first, K, st); break;
+    case 3: launch_resident_t<BD_MIRROR, true>(pl, L, parity, first, K, st); break;
+    case 4: launch_resident_t<BD_REFLECT, false>(pl, L, parity, first, K, st); break;
+    default: launch_resident_t<BD_REFLECT, true>(pl, L, parity, first, K, st); break;
+  }
+}
+
+template <int BC, bool ANISO>
+static int allow_resident_lds() {
+  const int v = 96 * 1024;
+  const void* kernels[] = {
+      (const void*)k_sweep_resident<BC, ANISO, false, 1>, (const void*)k_sweep_resident<BC, ANISO, true, 1>,
+      (const void*)k_sweep_resident<BC, ANISO, false, 2>, (const void*)k_sweep_resident<BC, ANISO, true, 2>};
+  for (const void* k : kernels) SVS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, v));
+  return 0;
+}
+
 // Dynamic LDS above 64 KiB must be enabled per kernel.
 static int allow_large_lds(const svs_plan* pl) {
   // shape_ok keeps every tile within 96 KiB; enable that much once.
@@ -1174,6 +1426,13 @@ static int allow_large_lds(const svs_plan* pl) {
       (const void*)k_sweep<BD_MIRROR, false, true>,   (const void*)k_sweep<BD_MIRROR, true, true>,
       (const void*)k_sweep<BD_REFLECT, false, true>,  (const void*)k_sweep<BD_REFLECT, true, true>};
   for (const void* k : kernels) SVS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, v));
+  // The resident sweep holds 48 bytes per voxel of up to 2048 voxels.
+  if (int rc = allow_resident_lds<BD_NEAREST, false>()) return rc;
+  if (int rc = allow_resident_lds<BD_NEAREST, true>()) return rc;
+  if (int rc = allow_resident_lds<BD_MIRROR, false>()) return rc;
+  if (int rc = allow_resident_lds<BD_MIRROR, true>()) return rc;
+  if (int rc = allow_resident_lds<BD_REFLECT, false>()) return rc;
+  if (int rc = allow_resident_lds<BD_REFLECT, true>()) return rc;
   return 0;
 }
 
@@ -1206,6 +1465,12 @@ static int enqueue_solve(svs_plan* pl, hipStream_t st) {
     for (int w = 0; w < P.max_warps; ++w) {
       k_linearise<<<L.nblocks, kBlock, 0, st>>>(L.dev, L.block_start, n, parity, ip, bc);
       SVS_LAUNCHED();
+      if (L.resident && P.max_outer_iterations > 0) {
+        launch_resident(pl, L, parity, 0, P.max_outer_iterations, st);
+        SVS_LAUNCHED();
+        parity ^= P.max_outer_iterations & 1;
+        continue;
+      }
       for (int it = 0; it < P.max_outer_iterations; ++it) {
         launch_sweep(pl, L, parity, it, st);
         SVS_LAUNCHED();
@@ -1346,7 +1611,7 @@ int svs_tvl1_plan_info(const svs_plan* pl, int level, int* out /* 8 ints */) {
   SVS_REQUIRE(pl && level >= 0 && level < pl->S && out, "bad plan/level");
   const LevelPlan& L = pl->levels[level];
   out[0] = L.bx; out[1] = L.by; out[2] = L.cz; out[3] = L.ntiles; out[4] = L.nblocks;
-  out[5] = pl->ninst; out[6] = pl->S; out[7] = 0;
+  out[5] = pl->ninst; out[6] = pl->S; out[7] = L.resident;
   return 0;
 }
 
@@ -1359,9 +1624,17 @@ int svs_tvl1_plan_sweeps(svs_plan* pl, int level, int count, void* stream) {
   SVS_REQUIRE(pl->P.max_outer_iterations > 0, "no iterations configured");
   hipStream_t st = (hipStream_t)stream;
   if (int rc = after_last_work(pl, st)) return rc;
+  const LevelPlan& L = pl->levels[level];
+  if (L.resident) {  // the level's iterations run in one resident launch
+    if (count > 0) {
+      launch_resident(pl, L, 0, 0, count, st);
+      SVS_LAUNCHED();
+    }
+    return mark_last_work(pl, st);
+  }
   int parity = 0;
   for (int i = 0; i < count; ++i) {
-    launch_sweep(pl, pl->levels[level], parity, i % pl->P.max_outer_iterations, st);
+    launch_sweep(pl, L, parity, i % pl->P.max_outer_iterations, st);
     SVS_LAUNCHED();
     parity ^= 1;
   }
@@ -1384,6 +1657,10 @@ int svs_tvl1_plan_autotune(svs_plan* pl, int reps, void* stream, int* report /* 
   int rc = 0;
   for (int s = 0; s < pl->S && !rc; ++s) {
     LevelPlan& L = pl->levels[s];
+    if (L.resident) {  // no marching sweep to tune at this level
+      if (report) { report[3 * s] = L.bx; report[3 * s + 1] = L.by; report[3 * s + 2] = L.cz; }
+      continue;
+    }
     auto time_shape = [&](int bx, int by, int cz, float* ms) -> int {
       L.bx = bx; L.by = by; L.cz = cz;
       if (int r = upload_tiles(pl, s, st)) return r;
@@ -1447,6 +1724,20 @@ int svs_tvl1_plan_set_shape(svs_plan* pl, int level, int bx, int by, int cz, voi
   if (int rc = after_last_work(pl, st)) return rc;
   L.bx = bx; L.by = by; L.cz = cz;
   if (int rc = upload_tiles(pl, level, st)) return rc;
+  if (pl->exec) { (void)hipGraphExecDestroy(pl->exec); pl->exec = nullptr; }
+  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  return mark_last_work(pl, st);
+}
+
+int svs_tvl1_plan_set_resident(svs_plan* pl, int level, int mode, void* stream) {
+  SVS_REQUIRE(pl && level >= 0 && level < pl->S, "bad plan/level");
+  SVS_REQUIRE(mode >= -1 && mode <= 1, "mode must be -1 (automatic), 0 (marching sweep) or 1 (resident)");
+  LevelPlan& L = pl->levels[level];
+  SVS_REQUIRE(mode != 1 || L.res_capable, "level %d: an instance exceeds the %d voxels the resident sweep holds",
+              level, 1024 * kResidentPerThread);
+  hipStream_t st = (hipStream_t)stream;
+  if (int rc = after_last_work(pl, st)) return rc;
+  L.resident = mode == 0 ? 0 : L.res_capable;
   if (pl->exec) { (void)hipGraphExecDestroy(pl->exec); pl->exec = nullptr; }
   if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
   return mark_last_work(pl, st);

@@ -135,7 +135,7 @@ class Plan:
     def info(self, level):
         out = (ctypes.c_int * 8)()
         check(_lib.lib().svs_tvl1_plan_info(self._h, level, out))
-        return {"bx": out[0], "by": out[1], "cz": out[2], "tiles": out[3], "blocks": out[4]}
+        return {"bx": out[0], "by": out[1], "cz": out[2], "tiles": out[3], "blocks": out[4], "resident": out[7]}
 
     def bind(self, k, I0, I1, u_out=None, p_out=None, u_in=None, p_in=None):
         """Binds instance k; the tensors must stay alive until the run completes."""
@@ -178,6 +178,13 @@ class Plan:
         with torch.cuda.device(self.device):
             for lv, (bx, by, cz) in enumerate(shapes):
                 check(_lib.lib().svs_tvl1_plan_set_shape(self._h, lv, int(bx), int(by), int(cz), stream_ptr()))
+
+    def set_resident(self, level, mode):
+        """How a level runs its iterations: 1 all of a warp's iterations in one
+        resident launch (levels of at most 2048 voxels per instance), 0 the
+        marching sweep once per iteration, -1 the automatic choice."""
+        with torch.cuda.device(self.device):
+            check(_lib.lib().svs_tvl1_plan_set_resident(self._h, int(level), int(mode), stream_ptr()))
 
     def sweeps(self, level, count):
         """Enqueues ``count`` fused sweeps of one level (timing aid; needs a prior run)."""

@@ -243,3 +243,61 @@ def test_set_shapes_keeps_results_and_rejects_bad_shapes(dev):
         plan.set_shapes([(40, 40, 8), (14, 6, 3)])      # more than 1024 threads
     with pytest.raises(RuntimeError):
         plan.set_shapes([(26, 6, 8)])                   # one shape per level
+
+
+@pytest.mark.parametrize("boundary", ["MIRROR", "NEAREST", "REFLECT"])
+@pytest.mark.parametrize("aniso", [True, False])
+def test_resident_levels_equal_marching_sweep(O, dev, boundary, aniso):
+    """The resident sweep (all iterations of a warp in one launch, one
+    work-group per instance) gives the marching sweep's bits: one voxel per
+    thread (8^3, 10^3 levels) and two (10x12x14 = 1680 voxels), in one batch;
+    and the oracle's."""
+    shapes = [(20, 24, 28), (16, 16, 16), (20, 20, 20)]
+    cfg = SolverConfig(num_scales=2, max_warps=2, max_outer_iterations=7, use_aniso=aniso,
+                       boundary={"NEAREST": 0, "MIRROR": 1, "REFLECT": 2}[boundary])
+    pairs = [pair(s, seed=50 + i) for i, s in enumerate(shapes)]
+    inputs = [(torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)) for a, b, _ in pairs]
+    outs = {}
+    for mode in (-1, 0):
+        plan = Plan(cfg, shapes)
+        if mode == 0:
+            plan.set_resident(1, 0)
+        assert plan.info(1)["resident"] == (2 if mode == -1 else 0)
+        assert plan.info(0)["resident"] == 0
+        us, ps = [], []
+        for k, (a, b) in enumerate(inputs):
+            u = torch.empty(shapes[k] + (3,), device=dev)
+            p = torch.empty(shapes[k] + (3, 3), device=dev)
+            plan.bind(k, a, b, u_out=u, p_out=p)
+            us.append(u)
+            ps.append(p)
+        plan.run(use_graph=True)
+        torch.cuda.synchronize()
+        outs[mode] = ([u.cpu().numpy() for u in us], [p.cpu().numpy() for p in ps])
+        plan.close()
+    for k in range(len(shapes)):
+        assert np.array_equal(outs[-1][0][k], outs[0][0][k]), f"u of instance {k}"
+        assert np.array_equal(outs[-1][1][k], outs[0][1][k]), f"p of instance {k}"
+    uo, po = oracle_solve(O, pairs[1][0], pairs[1][1], 2, 2, 7, boundary=boundary, aniso=aniso)
+    assert np.array_equal(outs[-1][0][1], uo) and np.array_equal(outs[-1][1][1], po)
+
+
+def test_resident_single_level_and_odd_iteration_count(O, dev):
+    """A one-level plan whose only level is resident, with an odd number of
+    iterations per warp (the state ends in the other buffer), equals the
+    oracle; forcing the resident mode on a level that is too large fails."""
+    shape = (11, 12, 13)
+    I0, I1, _ = pair(shape, seed=61)
+    solver = make_solver(1, 3, 5)
+    z = torch.zeros(shape + (3,), device=dev)
+    zp = torch.zeros(shape + (3, 3), device=dev)
+    u, p = solver.compute(torch.from_numpy(I0).to(dev), torch.from_numpy(I1).to(dev), z, zp)
+    plan = next(iter(solver._plans.values()))
+    assert plan.info(0)["resident"] == 2
+    w = gaussian_weights3d(1.0)
+    uo, po = O.solve(O.blur(I0, w), O.blur(I1, w), O.solve_params(num_scales=1, max_warps=3, max_outer_iterations=5))
+    assert np.array_equal(u.cpu().numpy(), uo) and np.array_equal(p.cpu().numpy(), po)
+    big = Plan(SolverConfig(num_scales=1, max_warps=1, max_outer_iterations=2), [(16, 16, 16)])
+    with pytest.raises(RuntimeError):
+        big.set_resident(0, 1)
+    big.close()
