@@ -351,7 +351,9 @@ def serve_leg(args, dev):
               "p50_ms_pairs": ms(pairs, 50), "p99_ms_pairs": ms(pairs, 99),
               "p50_ms_sequences": ms(seqs, 50), "p99_ms_sequences": ms(seqs, 99),
               "offered_solves_per_s": round(offered, 2), "capacity_solves_per_s": round(capacity, 2),
-              "offered_over_capacity": round(offered / capacity, 3)}
+              "offered_over_capacity": round(offered / capacity, 3),
+              "policy": {"slots": len(server.streams), "sequence_slots": server.sequence_slots,
+                         "pairs_first": server.pairs_first, "pair_buckets": list(buckets)}}
     server.close()
     log(args, f"serve: {served}")
 

@@ -44,6 +44,7 @@ class Request:
     # results
     flows: Optional[torch.Tensor] = None          # pair [Z,Y,X,3]; sequence [2(T-1),Z,Y,X,3]
     propagated: Optional[Tuple[torch.Tensor, torch.Tensor]] = None
+    admitted: Optional[float] = None   # server clock when its unit was admitted
     done: Optional[float] = None
 
     @property
@@ -72,8 +73,19 @@ def _bucket(n, buckets):
 
 class FlowServer:
     def __init__(self, cfg: SolverConfig, device=None, hbm_budget_bytes: int = 64 << 30, slots: int = 4,
-                 pair_buckets=(1, 2, 4, 8, 16), use_graph: bool = True):
+                 pair_buckets=(1, 2, 4, 8, 16), use_graph: bool = True, sequence_slots: Optional[int] = -1,
+                 pairs_first: bool = True):
+        """``sequence_slots`` caps how many sequence units run at once, so the
+        other slots stay open to pairs (-1: half the slots; None: no cap);
+        ``pairs_first`` admits pending pairs before pending sequences instead
+        of in arrival order.  The defaults keep the capacity of arrival-order
+        admission and cut latency: at 4 req/s of the five-size mix (0.7 of
+        capacity) p50 338 -> 270 ms and p99 2.79 -> 2.47 s over all requests,
+        pairs' p99 1.60 -> 1.28 s, at the same 114-125 solves/s capacity (one
+        sequence at a time cuts the p99 further but costs 7 % of capacity)."""
         self.cfg = cfg
+        self.sequence_slots = max(1, slots // 2) if sequence_slots == -1 else sequence_slots
+        self.pairs_first = pairs_first
         dev = torch.device(device) if device is not None else torch.device("cuda")
         if dev.index is None:
             dev = torch.device("cuda", torch.cuda.current_device())
@@ -176,11 +188,17 @@ class FlowServer:
     def _next_unit(self) -> Optional[Tuple[List[Request], Shape, int]]:
         if not self.pending:
             return None
-        head = self.pending[0]
-        if head.kind == "sequence":
-            return [head], head.shape, head.solves
-        group = [r for r in self.pending if r.kind == "pair" and r.shape == head.shape][:self.pair_buckets[-1]]
-        return group, head.shape, _bucket(len(group), self.pair_buckets)
+        pairs = [r for r in self.pending if r.kind == "pair"]
+        seq = next((r for r in self.pending if r.kind == "sequence"), None)
+        running_seq = sum(1 for u in self.running if u.requests[0].kind == "sequence")
+        seq_ok = seq is not None and (self.sequence_slots is None or running_seq < self.sequence_slots)
+        if pairs and (self.pairs_first or not seq_ok or self.pending[0].kind == "pair"):
+            head = pairs[0]
+            group = [r for r in pairs if r.shape == head.shape][:self.pair_buckets[-1]]
+            return group, head.shape, _bucket(len(group), self.pair_buckets)
+        if seq_ok:
+            return [seq], seq.shape, seq.solves
+        return None
 
     def _admit(self, now: float) -> bool:
         if not self.free_slots:
@@ -201,6 +219,7 @@ class FlowServer:
             return False
         for r in reqs:
             self.pending.remove(r)
+            r.admitted = now
         slot = self.free_slots.popleft()
         stream = self.streams[slot]
         plan, key = self._plan_for(shape, count, out_bytes)
