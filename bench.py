@@ -61,11 +61,125 @@ def parse():
     ap.add_argument("--serve-burst", type=int, default=30, help="serving leg: requests of the capacity burst")
     ap.add_argument("--serve-plain-requests", type=int, default=30,
                     help="serving leg: requests run through the plain per-request drop-in")
+    ap.add_argument("--no-live-traffic", action="store_true",
+                    help="take roofline.traffic from the committed counter summary instead of measuring it")
+    ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
 
 PMC_SUMMARY = os.path.join("profiles", "k_sweep_68x80cubed_pmc.md")
+
+
+def _run_child(cmd, timeout_s, cwd):
+    """Runs cmd in its own process group; kills the group at the time limit.
+    Returns (returncode or None on timeout, combined output)."""
+    import signal
+    import subprocess
+    env = dict(os.environ, TMPDIR="/tmp")
+    proc = subprocess.Popen(cmd, cwd=cwd, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                            start_new_session=True)
+    try:
+        out, _ = proc.communicate(timeout=timeout_s)
+        return proc.returncode, out
+    except subprocess.TimeoutExpired:
+        os.killpg(proc.pid, signal.SIGKILL)
+        out, _ = proc.communicate()
+        return None, out
+
+
+def _counter_rows(outdir, counter):
+    """(dispatch id, kernel name, value) of one counter from a rocprofv3 CSV
+    tree, and the CSV header seen (for the error message when nothing matches)."""
+    import csv
+    import glob
+    rows, header = [], None
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            reader = csv.DictReader(f)
+            header = reader.fieldnames
+            for r in reader:
+                if r.get("Counter_Name") == counter:
+                    rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return rows, header
+
+
+def live_pmc_traffic(args, shapes, resident, algorithmic):
+    """HBM bytes per fused-sweep iteration of every level, measured on the tuned
+    shapes: this script runs twice as a child under rocprofv3 --pmc (FETCH_SIZE,
+    then WRITE_SIZE) in its --pmc-probe mode, which builds the same plan, runs
+    --sweep-launches sweeps of each level (one launch of that many iterations at
+    a resident level) and a 1 GiB device copy.  The copy calibrates both
+    counters (bytes = counter x 1 GiB / the copy's counter).  Returns a dict, or
+    None and the reason."""
+    import shutil
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not found"
+    n = args.sweep_launches
+    spec = ";".join(",".join(str(v) for v in t) for t in shapes)
+    me = os.path.abspath(__file__)
+    per_counter = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        outdir = tempfile.mkdtemp(prefix="svs_pmc_", dir="/tmp")
+        cmd = [prof, "--pmc", counter, "-d", outdir, "-o", "run", "--output-format", "csv", "--",
+               sys.executable, me, "--pmc-probe", "--quiet", "--size", str(args.size), "--frames", str(args.frames),
+               "--sweep-launches", str(n), "--sweep-shapes", spec]
+        log(args, f"traffic: rocprofv3 --pmc {counter} on the sweeps of every level")
+        rc, out = _run_child(cmd, 240, outdir)
+        rows, header = _counter_rows(outdir, counter)
+        shutil.rmtree(outdir, ignore_errors=True)
+        if "pmc-probe done" not in (out or ""):
+            return None, f"rocprofv3 --pmc {counter} run did not complete (rc {rc})"
+        sweeps = [r for r in rows if "k_sweep" in r[1]]
+        copies = [r for r in rows if "copyBuffer" in r[1]]
+        counts = [1 if res else n for res in resident]
+        if len(sweeps) < sum(counts) or not copies:
+            return None, (f"{counter}: {len(sweeps)} sweep and {len(copies)} copy dispatches in the counter output "
+                          f"(columns {header})")
+        sweeps = sweeps[len(sweeps) - sum(counts):]
+        per_level, i = [], 0
+        for lv, c in enumerate(counts):
+            vals = [v for _, _, v in sweeps[i:i + c]]
+            i += c
+            per_level.append(sum(vals) / len(vals) / (n if resident[lv] else 1))  # per iteration
+        per_counter[counter] = (per_level, copies[-1][2])
+    gib = float(1 << 30)
+    fetch, f_copy = per_counter["FETCH_SIZE"]
+    write, w_copy = per_counter["WRITE_SIZE"]
+    levels = [f * gib / f_copy + w * gib / w_copy for f, w in zip(fetch, write)]
+    return {"traffic": int(sum(levels) / len(levels)), "per_level": [int(v) for v in levels],
+            "per_level_over_algorithmic": [round(v / a, 3) for v, a in zip(levels, algorithmic)],
+            "FETCH_SIZE_per_level": [round(v, 1) for v in fetch], "WRITE_SIZE_per_level": [round(v, 1) for v in write],
+            "calibration": {"copy_bytes": int(gib), "FETCH_SIZE_copy": f_copy, "WRITE_SIZE_copy": w_copy}}, None
+
+
+def pmc_probe(args):
+    """--pmc-probe: the sweeps live_pmc_traffic counts (run under rocprofv3)."""
+    import torch
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from singleventriclesegmentation_amd.solver import Plan, SolverConfig
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    shape = (args.size,) * 3
+    npairs = 2 * (args.frames - 1)
+    cfg = SolverConfig(num_scales=4, max_warps=1, max_outer_iterations=2)
+    frames = torch.rand((2,) + shape, device=dev)
+    plan = Plan(cfg, [shape] * npairs, device=dev)
+    for k in range(npairs):
+        plan.bind(k, frames[0], frames[1])
+    plan.run(use_graph=False)
+    plan.set_shapes([tuple(int(v) for v in t.split(",")) for t in args.sweep_shapes.split(";")])
+    for lv in range(cfg.num_scales):
+        plan.sweeps(lv, args.sweep_launches)
+    src = torch.empty(1 << 28, dtype=torch.float32, device=dev).uniform_()
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    plan.close()
+    print("pmc-probe done", flush=True)
 
 
 def pmc_traffic(size, npairs):
@@ -94,6 +208,8 @@ def log(args, *a):
 
 def main():
     args = parse()
+    if args.pmc_probe:
+        return pmc_probe(args)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}, WORLD_SIZE {world}): frame pairs are "
@@ -183,7 +299,29 @@ def main():
     avg_bytes = sum(sweep_bytes) / len(levels)
     avg_ms = sum(sweep_ms) / len(levels)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-    traffic, traffic_shapes = pmc_traffic(args.size, npairs)
+    traffic = None
+    traffic_detail = None
+    traffic_source = None
+    if not args.no_live_traffic:
+        resident = [plan.info(lv)["resident"] > 0 for lv in range(len(levels))]
+        live, why = live_pmc_traffic(args, tuned, resident, [b for b in sweep_bytes])
+        if live is not None:
+            traffic = live["traffic"]
+            traffic_detail = live
+            traffic_source = ("measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the tuned sweep "
+                              "shapes, each scaled by what it reads for a 1 GiB device copy; per iteration, mean "
+                              "of the levels")
+            log(args, "traffic per level (bytes per iteration):", live["per_level"],
+                "over algorithmic:", live["per_level_over_algorithmic"], "; counters per iteration FETCH_SIZE",
+                live["FETCH_SIZE_per_level"], "WRITE_SIZE", live["WRITE_SIZE_per_level"], "; 1 GiB copy",
+                live["calibration"])
+        else:
+            log(args, f"traffic: not measured live ({why})")
+    if traffic is None:
+        traffic, traffic_shapes = pmc_traffic(args.size, npairs)
+        if traffic:
+            traffic_source = (f"{PMC_SUMMARY}: 2 x FETCH_SIZE + WRITE_SIZE per launch, mean of the levels, "
+                              f"sweep shapes {traffic_shapes}")
     level0_gbps = sweep_bytes[0] / (sweep_ms[0] * 1e-3) / 1e9
     log(args, "sweep per level (us/launch):", [round(1e3 * v, 1) for v in sweep_ms])
 
@@ -296,9 +434,12 @@ def main():
         "algorithmic_GBps": round(solves_per_s * algorithmic_bytes(cfg, shape) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                     "traffic_source": (f"{PMC_SUMMARY}: 2 x FETCH_SIZE + WRITE_SIZE per launch, mean of the "
-                                        f"levels, sweep shapes {traffic_shapes}") if traffic else None,
-                     "kernel": "k_sweep (fused primal-dual sweep)",
+                     "traffic_source": traffic_source,
+                     "traffic_per_level": traffic_detail["per_level"] if traffic_detail else None,
+                     "traffic_over_algorithmic_per_level": (traffic_detail["per_level_over_algorithmic"]
+                                                            if traffic_detail else None),
+                     "kernel": "k_sweep (fused primal-dual sweep; k_sweep_resident, all iterations of a warp in "
+                               "one launch, at levels of at most 2048 voxels per instance)",
                      "bytes_per_launch_avg": int(avg_bytes), "us_per_launch_avg": round(avg_ms * 1e3, 2),
                      "level0_GBps": round(level0_gbps, 1),
                      "us_per_launch_by_level": [round(v * 1e3, 2) for v in sweep_ms]},
