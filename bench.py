@@ -554,7 +554,11 @@ def bench_uneven(args, dev):
     for k, sh in enumerate(shapes):
         plan.bind(k, frames[sh][0], frames[sh][1], u_out=outs[k])
     plan.run(use_graph=True)
-    tuned = plan.autotune()
+    if args.sweep_shapes:
+        tuned = [tuple(int(v) for v in t.split(",")) for t in args.sweep_shapes.split(";")]
+        plan.set_shapes(tuned)
+    else:
+        tuned = plan.autotune(reps=5)
     for _ in range(args.warmup):
         plan.run(use_graph=True)
     torch.cuda.synchronize()

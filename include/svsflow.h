@@ -258,8 +258,11 @@ int svs_tvl1_plan_sweeps(svs_plan* plan, int level, int count, void* stream);
 
 /* Autotuner: times candidate fused-sweep shapes (work-group tile and
  * z-chunk) per level on the plan's state (needs one prior run), keeps the
- * fastest, remembers it for later plans of the same geometry in this
- * process and drops the captured graph.  report (optional) receives
+ * fastest; at levels that qualify for the resident sweep it also times one
+ * resident launch of a warp's iterations against that many marching sweeps
+ * and keeps the faster per iteration (iterations per launch: K or 1).  It
+ * remembers both for later plans of the same geometry in this process and
+ * drops the captured graph.  report (optional) receives
  * bx, by, cz per level.  Results do not depend on the shape.  No reference
  * counterpart. */
 int svs_tvl1_plan_autotune(svs_plan* plan, int reps, void* stream, int* report);

@@ -975,6 +975,7 @@ static std::vector<int> cz_candidates(const LevelPlan& L) {
 // kernel variant, so later plans of the same geometry start tuned.
 static std::mutex g_tune_mu;
 static std::map<std::string, SweepShape> g_tuned;
+static std::map<std::string, bool> g_tuned_resident;  // levels that qualify: resident faster?
 
 static std::string tune_key(const svs_tvl1_params& P, const LevelPlan& L) {
   std::map<std::array<int, 3>, int> hist;
@@ -1110,7 +1111,12 @@ static int plan_layout(svs_plan* pl, char* base, size_t* total_out) {
         L.res_threads = std::min(1024, (per + 63) / 64 * 64);
         L.res_lds = sizeof(float) * 12 * (size_t)maxN;
       }
-      L.resident = L.res_capable;  // automatic: resident whenever the level qualifies
+      L.resident = L.res_capable;  // automatic: resident whenever the level qualifies ...
+      if (L.res_capable) {         // ... unless the autotuner measured the marching sweep faster
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned_resident.find(tune_key(P, L));
+        if (it != g_tuned_resident.end() && !it->second) L.resident = 0;
+      }
     }
     L.px_blocks = L.py_blocks = 0;
     if (s + 1 < S)
@@ -1260,6 +1266,7 @@ static int upload_static(svs_plan* pl, hipStream_t st) {
     for (int k = 0; k < pl->ninst; ++k) { bs[k] = acc; acc += grid_for(L.host[k].N, kBlock); }
     bs[pl->ninst] = acc;
     SVS_HIP(hipMemcpyAsync(L.block_start, bs.data(), sizeof(int) * bs.size(), hipMemcpyHostToDevice, st));
+
     SVS_HIP(hipStreamSynchronize(st));
     if (s + 1 < pl->S) {
       std::vector<int> bx(pl->ninst + 1), by(pl->ninst + 1);
@@ -1657,10 +1664,6 @@ int svs_tvl1_plan_autotune(svs_plan* pl, int reps, void* stream, int* report /* 
   int rc = 0;
   for (int s = 0; s < pl->S && !rc; ++s) {
     LevelPlan& L = pl->levels[s];
-    if (L.resident) {  // no marching sweep to tune at this level
-      if (report) { report[3 * s] = L.bx; report[3 * s + 1] = L.by; report[3 * s + 2] = L.cz; }
-      continue;
-    }
     auto time_shape = [&](int bx, int by, int cz, float* ms) -> int {
       L.bx = bx; L.by = by; L.cz = cz;
       if (int r = upload_tiles(pl, s, st)) return r;
@@ -1676,7 +1679,7 @@ int svs_tvl1_plan_autotune(svs_plan* pl, int reps, void* stream, int* report /* 
     };
     // Stage 1: every tile with its default z-chunk.  Stage 2: every z-chunk
     // for the five fastest tiles of stage 1 (shapes a few per cent apart
-    // change rank between repetitions).
+    // change rank between repetitions).  Stage 3 below re-times the best.
     SweepShape best{L.bx, L.by, L.cz};
     float best_ms = 0.0f;
     if ((rc = time_shape(best.bx, best.by, best.cz, &best_ms))) break;
@@ -1690,20 +1693,71 @@ int svs_tvl1_plan_autotune(svs_plan* pl, int reps, void* stream, int* report /* 
     if (rc) break;
     std::sort(ranked.begin(), ranked.end(),
               [](const std::pair<float, SweepShape>& a, const std::pair<float, SweepShape>& b) { return a.first < b.first; });
+    std::vector<std::pair<float, SweepShape>> timed = ranked;
     for (size_t r = 0; r < ranked.size() && r < 5 && !rc; ++r) {
       const SweepShape t = ranked[r].second;
       for (int cz : cz_candidates(L)) {
         float ms = 0.0f;
         if ((rc = time_shape(t.bx, t.by, cz, &ms))) break;
+        timed.push_back({ms, SweepShape{t.bx, t.by, cz}});
         if (ms < best_ms) { best_ms = ms; best = SweepShape{t.bx, t.by, cz}; }
       }
     }
     if (rc) break;
+    // Stage 3: the four fastest distinct shapes of stages 1 and 2 timed again,
+    // twice each in alternating order, so that one lucky repetition does not
+    // decide between shapes a few per cent apart.
+    std::sort(timed.begin(), timed.end(),
+              [](const std::pair<float, SweepShape>& a, const std::pair<float, SweepShape>& b) { return a.first < b.first; });
+    std::vector<SweepShape> finalists;
+    for (auto& e : timed) {
+      bool dup = false;
+      for (auto& f : finalists) dup |= f.bx == e.second.bx && f.by == e.second.by && f.cz == e.second.cz;
+      if (!dup) finalists.push_back(e.second);
+      if (finalists.size() == 4) break;
+    }
+    std::vector<float> total(finalists.size(), 0.0f);
+    for (int round = 0; round < 2 && !rc; ++round)
+      for (size_t f = 0; f < finalists.size() && !rc; ++f) {
+        const size_t i = round ? finalists.size() - 1 - f : f;
+        float ms = 0.0f;
+        if ((rc = time_shape(finalists[i].bx, finalists[i].by, finalists[i].cz, &ms))) break;
+        total[i] += ms;
+      }
+    if (rc) break;
+    size_t win = 0;
+    for (size_t f = 1; f < finalists.size(); ++f)
+      if (total[f] < total[win]) win = f;
+    best = finalists[win];
+    best_ms = total[win] / 2;
     L.bx = best.bx; L.by = best.by; L.cz = best.cz;
     if ((rc = upload_tiles(pl, s, st))) break;
     if (report) { report[3 * s] = best.bx; report[3 * s + 1] = best.by; report[3 * s + 2] = best.cz; }
+    // A level that qualifies for the resident sweep: one launch of a warp's
+    // K iterations against K marching sweeps of the best shape, per iteration.
+    bool resident_wins = false;
+    if (L.res_capable) {
+      const int K = pl->P.max_outer_iterations;
+      auto time_resident = [&](float* ms) -> int {
+        launch_resident(pl, L, 0, 0, K, st);
+        SVS_LAUNCHED();
+        SVS_HIP(hipEventRecord(e0, st));
+        launch_resident(pl, L, 0, 0, K, st);
+        SVS_LAUNCHED();
+        SVS_HIP(hipEventRecord(e1, st));
+        SVS_HIP(hipEventSynchronize(e1));
+        SVS_HIP(hipEventElapsedTime(ms, e0, e1));
+        return 0;
+      };
+      float ms_res = 0.0f;
+      L.resident = L.res_capable;
+      if ((rc = time_resident(&ms_res))) break;
+      resident_wins = ms_res / K < best_ms / reps;
+      L.resident = resident_wins ? L.res_capable : 0;
+    }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[tune_key(pl->P, L)] = best;
+    if (L.res_capable) g_tuned_resident[tune_key(pl->P, L)] = resident_wins;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
