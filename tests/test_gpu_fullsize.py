@@ -206,3 +206,41 @@ def test_config3_serving_fullsize(dev):
                        r.flows[0:12], r.flows[T - 1:T - 1 + 12].flip(0))
     assert torch.equal(fm, r.propagated[0]) and torch.equal(bm, r.propagated[1])
     server.close()
+
+
+def test_instance_at_the_size_limit(dev):
+    """An instance at the plan's size limit (488 x 488 x 500 = 119,072,000
+    voxels: its nine dual fields span 36 N bytes, just below the 2^32 the
+    sweep's 32-bit offsets address) solved in one plan with a small instance
+    placed after it in the workspace: identical frames give exactly zero flow
+    and dual (as the oracle gives at any size), and the small instance equals
+    its solve alone."""
+    big, small = (500, 488, 488), (20, 24, 28)
+    assert 36 * big[0] * big[1] * big[2] < 2 ** 32
+    cfg = SolverConfig(num_scales=2, max_warps=1, max_outer_iterations=2)
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    frame = torch.rand(big, device=dev, generator=g)
+    f0, f1, _ = phantom.frame_pair(small, seed0=70, seed1=71, disp_seed=72, peak=2.0)
+    a, b = torch.from_numpy(f1).to(dev), torch.from_numpy(f0).to(dev)
+    u_big = torch.full(big + (3,), 7.0, device=dev)
+    p_big = torch.full(big + (3, 3), 7.0, device=dev)
+    u_s = torch.empty(small + (3,), device=dev)
+    p_s = torch.empty(small + (3, 3), device=dev)
+    plan = Plan(cfg, [big, small], device=dev)
+    plan.bind(0, frame, frame, u_out=u_big, p_out=p_big)
+    plan.bind(1, a, b, u_out=u_s, p_out=p_s)
+    plan.run(use_graph=False)
+    torch.cuda.synchronize()
+    assert int(torch.count_nonzero(u_big)) == 0 and int(torch.count_nonzero(p_big)) == 0
+    plan.close()
+    del u_big, p_big, frame
+    alone = Plan(cfg, [small], device=dev)
+    u_a = torch.empty(small + (3,), device=dev)
+    p_a = torch.empty(small + (3, 3), device=dev)
+    alone.bind(0, a, b, u_out=u_a, p_out=p_a)
+    alone.run(use_graph=False)
+    torch.cuda.synchronize()
+    assert torch.equal(u_a, u_s) and torch.equal(p_a, p_s)
+    assert float(u_s.abs().max()) > 0.01   # the small pair does move
+    alone.close()
