@@ -63,6 +63,8 @@ def parse():
                     help="serving leg: requests run through the plain per-request drop-in")
     ap.add_argument("--no-live-traffic", action="store_true",
                     help="take roofline.traffic from the committed counter summary instead of measuring it")
+    ap.add_argument("--pmc-timeout", type=int, default=90,
+                    help="seconds allowed for each child rocprofv3 counter run (the roofline traffic)")
     ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
@@ -128,7 +130,7 @@ def live_pmc_traffic(args, shapes, resident, algorithmic):
                sys.executable, me, "--pmc-probe", "--quiet", "--size", str(args.size), "--frames", str(args.frames),
                "--sweep-launches", str(n), "--sweep-shapes", spec]
         log(args, f"traffic: rocprofv3 --pmc {counter} on the sweeps of every level")
-        rc, out = _run_child(cmd, 240, outdir)
+        rc, out = _run_child(cmd, args.pmc_timeout, outdir)
         rows, header = _counter_rows(outdir, counter)
         shutil.rmtree(outdir, ignore_errors=True)
         if "pmc-probe done" not in (out or ""):
