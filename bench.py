@@ -107,6 +107,26 @@ def _counter_rows(outdir, counter):
     return rows, header
 
 
+def per_level_counter(rows, resident, n):
+    """Per-iteration counter value of every level and the 1 GiB copy's value,
+    from (dispatch id, kernel name, value) rows in dispatch order: the last
+    sweep dispatches are the probe's n sweeps per level in level order (one
+    n-iteration launch at a resident level); the last copy is the calibration
+    copy.  None when the rows hold fewer."""
+    sweeps = [r for r in rows if "k_sweep" in r[1]]
+    copies = [r for r in rows if "copyBuffer" in r[1]]
+    counts = [1 if res else n for res in resident]
+    if len(sweeps) < sum(counts) or not copies:
+        return None
+    sweeps = sweeps[len(sweeps) - sum(counts):]
+    per_level, i = [], 0
+    for lv, c in enumerate(counts):
+        vals = [v for _, _, v in sweeps[i:i + c]]
+        i += c
+        per_level.append(sum(vals) / len(vals) / (n if resident[lv] else 1))
+    return per_level, copies[-1][2]
+
+
 def live_pmc_traffic(args, shapes, resident, algorithmic):
     """HBM bytes per fused-sweep iteration of every level, measured on the tuned
     shapes: this script runs twice as a child under rocprofv3 --pmc (FETCH_SIZE,
@@ -135,19 +155,10 @@ def live_pmc_traffic(args, shapes, resident, algorithmic):
         shutil.rmtree(outdir, ignore_errors=True)
         if "pmc-probe done" not in (out or ""):
             return None, f"rocprofv3 --pmc {counter} run did not complete (rc {rc})"
-        sweeps = [r for r in rows if "k_sweep" in r[1]]
-        copies = [r for r in rows if "copyBuffer" in r[1]]
-        counts = [1 if res else n for res in resident]
-        if len(sweeps) < sum(counts) or not copies:
-            return None, (f"{counter}: {len(sweeps)} sweep and {len(copies)} copy dispatches in the counter output "
-                          f"(columns {header})")
-        sweeps = sweeps[len(sweeps) - sum(counts):]
-        per_level, i = [], 0
-        for lv, c in enumerate(counts):
-            vals = [v for _, _, v in sweeps[i:i + c]]
-            i += c
-            per_level.append(sum(vals) / len(vals) / (n if resident[lv] else 1))  # per iteration
-        per_counter[counter] = (per_level, copies[-1][2])
+        got = per_level_counter(rows, resident, n)
+        if got is None:
+            return None, f"{counter}: too few sweep or copy dispatches in the counter output (columns {header})"
+        per_counter[counter] = got
     gib = float(1 << 30)
     fetch, f_copy = per_counter["FETCH_SIZE"]
     write, w_copy = per_counter["WRITE_SIZE"]
