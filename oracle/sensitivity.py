@@ -69,7 +69,8 @@ def main(argv=None):
     shape = (args.size,) * 3
     f0, f1, disp = phantom.frame_pair(shape)
     params = O.solve_params(num_scales=S, max_warps=W, max_outer_iterations=K)
-    subprocess.run(["make", "-s", "-C", HERE, "all", "variants"], check=True)
+    names = args.variants.split(",") if args.variants else []
+    subprocess.run(["make", "-s", "-C", HERE, "all"] + [f"_variants/libsvs_oracle_{v}.so" for v in names], check=True)
 
     O.use_library(None)
     t0 = time.perf_counter()
@@ -87,7 +88,7 @@ def main(argv=None):
 
     row("fp32", "default FP32 oracle (the GPU path's arithmetic)", base, t_base)
     row("fp64", "FP64 twin of the same iteration", f64, 0.0)
-    for name in args.variants.split(","):
+    for name in names:
         O.use_library(os.path.join(HERE, "_variants", f"libsvs_oracle_{name}.so"))
         t0 = time.perf_counter()
         u = solve((f0, f1), params)
