@@ -41,7 +41,7 @@ def diff_stats(a, b):
             "voxels_above_1e-3": int((d.amax(-1) > 1e-3).sum().item())}
 
 
-def test_config0_operator_path_80cubed(dev):
+def test_reference_driver_equals_fused_80cubed(dev):
     """The reference's step-by-step driver over the drop-in operators (one
     operator call per reference call, torch's own elementwise and sum
     kernels between them) gives the fused solve's bits at the benchmark size."""
@@ -79,7 +79,7 @@ def _oracle_chain(O, mask, flows):
     return np.stack(out)
 
 
-def test_config1_sequence_80cubed(O, dev):
+def test_sequence_35x80cubed_batch_and_propagation(O, dev):
     T, shape, last = 35, (80, 80, 80), 12
     seq, flows = _sequence_flows(dev, T, shape)
 
@@ -128,7 +128,7 @@ def test_config1_sequence_80cubed(O, dev):
     assert torch.equal(fm[0, 0], m0[0, 0]) and torch.equal(bm[last, 0], m12[0, 0])
 
 
-def test_config2_uneven_batch(O, dev):
+def test_uneven_five_size_batch(O, dev):
     sizes = list(phantom.UNEVEN_SIZES)
     pairs = []
     for i, sh in enumerate(sizes):
@@ -171,7 +171,7 @@ def test_config2_uneven_batch(O, dev):
     plan.close()
 
 
-def test_config3_serving_fullsize(dev):
+def test_serving_mix_with_80cubed_sequence(dev):
     """Continuous batching with an 80^3 sequence (68 solves + propagation)
     and 128x128x48 / 48^3 pairs arriving while it runs: every result equals
     the plain drop-in's alone."""

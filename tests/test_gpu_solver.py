@@ -157,7 +157,7 @@ def test_single_scale_uses_initial_state(O, dev):
 
 
 @pytest.mark.slow
-def test_config0_80cube_matches_oracle(O, dev):
+def test_pair_80cubed_matches_oracle(O, dev):
     """The benchmark case: 80^3 phantom pair, 4 levels x 10 warps x 50 iterations."""
     f0, f1, d = phantom.frame_pair((80, 80, 80))
     solver = make_solver(4, 10, 50)
