@@ -111,10 +111,14 @@ def check(rc):
         raise RuntimeError(msg or f"libsvsflow call failed with status {rc}")
 
 
-# torch's raw current-stream query (what its own generated code calls); it
-# skips building a Stream object per call, which dominated the host time of
-# the operator-by-operator path.  Older builds without it use the public API.
+# torch's raw current-stream and current-device queries (what its own
+# generated code calls); they skip building a Stream object and re-checking
+# lazy initialisation per call, which dominated the host time of the
+# operator-by-operator path.  Builds without them use the public API.  Every
+# caller holds a CUDA tensor already, so torch's CUDA state is initialised.
 _raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_raw_device = getattr(torch._C, "_cuda_getDevice", None) or torch.cuda.current_device
+_SAME_DEVICE = contextlib.nullcontext()
 
 
 def stream_ptr(device=None):
@@ -122,7 +126,7 @@ def stream_ptr(device=None):
     address for a c_void_p argument."""
     if _raw_stream is not None:
         if device is None or (isinstance(device, torch.device) and device.index is None):
-            idx = torch.cuda.current_device()
+            idx = _raw_device()
         else:
             idx = device.index if isinstance(device, torch.device) else int(device)
         return _raw_stream(idx)
@@ -133,9 +137,10 @@ def on_device(device):
     """Context that makes ``device`` current for the launches inside it; a
     no-op when it already is (the common case, and a measurable share of an
     operator call's host time otherwise)."""
-    idx = device.index if device.index is not None else torch.cuda.current_device()
-    if idx == torch.cuda.current_device():
-        return contextlib.nullcontext()
+    cur = _raw_device()
+    idx = device.index if device.index is not None else cur
+    if idx == cur:
+        return _SAME_DEVICE
     return torch.cuda.device(idx)
 
 
