@@ -178,10 +178,6 @@ __device__ __forceinline__ Chan chan_at(const AugArgs& a, int unit, int comp, si
   return c;
 }
 
-// Buffer offsets at or past 2^31 are outside every buffer of this pass: a
-// load there returns 0 and a store there is dropped (buffer range checking).
-constexpr uint32_t kOutside = 0x80000000u;
-
 __device__ __forceinline__ void store_vector(const AugArgs& a, svs_rsrc dst, uint32_t off, uint32_t comp_bytes,
                                              float v0, float v1, float v2) {
   float r0 = dot3(a.vector_rounding, a.R[0], v0, a.R[1], v1, a.R[2], v2);

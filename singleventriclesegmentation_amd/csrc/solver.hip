@@ -440,10 +440,13 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   // This thread's position on the outer ring of the u tile (one per thread;
   // the host guarantees 2*UW + 2*BY <= BX*BY).
   const int nring = 2 * UW + 2 * BY;
+  // Lanes off the ring, or with a ring position outside the volume, load
+  // from past the buffer and get 0 without a memory access: the ring loads
+  // then need no branch (a divergent one around them made the compiler merge
+  // their wave offsets into VGPRs and read them back lane by lane).
   const bool ring = tid < nring;
   int ring_lds = 0;
-  uint32_t ring_vo = 0;
-  bool ring_ok = false;
+  uint32_t ring_vo = kOutside;
   if (ring) {
     int ux, uy;
     if (tid < 2 * UW) {
@@ -455,8 +458,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
       ux = (rr & 1) ? UW - 1 : 0;
     }
     const int gx = (t.xy & 0xffff) - 2 + ux, gy = (t.xy >> 16) - 2 + uy;
-    ring_ok = gx >= 0 && gx < NX && gy >= 0 && gy < NY;
-    ring_vo = ring_ok ? (uint32_t)(gy * NX + gx) * 4u : 0u;
+    if (gx >= 0 && gx < NX && gy >= 0 && gy < NY) ring_vo = (uint32_t)(gy * NX + gx) * 4u;
     ring_lds = uy * UW + ux;
   }
 
@@ -482,8 +484,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
       snc[0] = sn.x; snc[1] = sn.y; snc[2] = sn.z; snc[3] = sn.w;
     }
   }
-  if (ring_ok)
-    for (int c = 0; c < 3; ++c) ringc[c] = bld(uin, ring_vo, c * F4 + kstart * P4);
+  for (int c = 0; c < 3; ++c) ringc[c] = bld(uin, ring_vo, c * F4 + kstart * P4);
   float r2a[3] = {0.0f, 0.0f, 0.0f}, r2b[3] = {0.0f, 0.0f, 0.0f}, r2c[3];
   // Each step reads this thread's u of the previous plane back from its own
   // slot of the other LDS buffer (only this thread writes that slot), so no
@@ -568,9 +569,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
     }
 
     for (int c = 0; c < 3; ++c) un[c] = bld(uin, vo, c * F4 + k2 * P4);
-    for (int c = 0; c < 3; ++c) ringn[c] = 0.0f;
-    if (ring)
-      for (int c = 0; c < 3; ++c) ringn[c] = bld(uin, ring_vo, c * F4 + k1 * P4);
+    for (int c = 0; c < 3; ++c) ringn[c] = bld(uin, ring_vo, c * F4 + k1 * P4);
     for (int e = 0; e < 9; ++e) pn[e] = bld(pin, vo, e * F4 + k1 * P4);
 
     // ---- primal step at plane k-1 on the tile interior

@@ -284,12 +284,16 @@ __device__ __forceinline__ float cd_adj(float pm, float p0, float pp, int i, int
 
 // anisotropicDifferentialOps.cu:144-175 applied to one row q (3 derivatives):
 // out_d = s (n.q) n_d + (t1.q) t1_d.  The reference's t2 term is multiplied by
-// a factor that stays 0 and is therefore omitted.
+// a factor that stays 0 and is therefore omitted, and its dot products start
+// from the first product instead of 0 + (the compiler must keep an add of +0,
+// which turns -0 into +0).  Both change at most the sign of a zero, and no
+// step of the solve lets the sign of a zero reach a nonzero value (every
+// divisor is positive), so the results are the same values.
 __device__ __forceinline__ void aniso_row(float s, const float n[3], const float t1[3], const float q[3],
                                           float out[3]) {
-  float nf = 0.0f, tf = 0.0f;
+  float nf = n[0] * q[0], tf = t1[0] * q[0];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 1; d < 3; ++d) {
     nf += n[d] * q[d];
     tf += t1[d] * q[d];
   }
@@ -354,19 +358,31 @@ typedef float svs_f4 __attribute__((ext_vector_type(4)));
 // size in bytes) sits in scalar registers; each access adds a per-lane byte
 // offset (VGPR) and a wave-uniform byte offset (SGPR), so the plane and field
 // offsets cost no vector instructions.  Callers keep every access inside
-// [base, base + bytes).  0x00020000 is the gfx9 word 3 (32-bit data format).
+// [base, base + bytes) or at kOutside.  0x00020000 is the gfx9 word 3 (32-bit
+// data format).
 typedef __amdgpu_buffer_rsrc_t svs_rsrc;
 __device__ __forceinline__ svs_rsrc make_rsrc(const void* base, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
 }
+// Buffer offsets at or past 2^31 are outside every buffer these kernels
+// address (each is smaller than 2^31 bytes, and offset plus wave offset stays
+// below 2^32): a load there returns 0 and a store there is dropped (buffer
+// range checking), with no memory access.
+constexpr uint32_t kOutside = 0x80000000u;
+
+// The wave offset goes through readfirstlane: the value is uniform by the
+// callers' contract, and the (convergent) builtin keeps the compiler from
+// merging equal offsets computed under different branches into a VGPR, which
+// it would then read back lane by lane around every access.
+__device__ __forceinline__ int wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane((int)v); }
 __device__ __forceinline__ float bld(svs_rsrc r, uint32_t voff, uint32_t soff) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, wave_uniform(soff), 0));
 }
 __device__ __forceinline__ svs_f4 bld4(svs_rsrc r, uint32_t voff, uint32_t soff) {
-  return __builtin_bit_cast(svs_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+  return __builtin_bit_cast(svs_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, wave_uniform(soff), 0));
 }
 __device__ __forceinline__ void bst(svs_rsrc r, uint32_t voff, uint32_t soff, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, (int)soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, wave_uniform(soff), 0);
 }
 
 // Linear voxel index -> (z, y, x).
