@@ -350,8 +350,14 @@ def step_schedule(params):
     return sig, tau
 
 
-def solve(I0, I1, params, u=None, p=None, dtype=np.float32):
-    """TVL13DOpticalFlow.compute on already-blurred I0, I1 (tvl13d.py:122-162)."""
+def solve(I0, I1, params, u=None, p=None, dtype=np.float32, stats=False):
+    """TVL13DOpticalFlow.compute on already-blurred I0, I1 (tvl13d.py:122-162).
+
+    With ``stats`` it also returns float64 [num_scales, max_warps,
+    max_outer_iterations, 3] per-iteration statistics (level 0 the finest):
+    ||u^{n+1} - u^n||, ||p^{n+1} - p^n|| (the reference's residual norms,
+    tvl13d.py:222, 237) and the energy E(u^n) of the iterate entering the
+    iteration (svs_oracle.cpp, IterStats)."""
     I0, I1 = _arr(I0, dtype), _arr(I1, dtype)
     NZ, NY, NX = I0.shape
     u = np.zeros((NZ, NY, NX, 3), dtype) if u is None else _arr(u, dtype)
@@ -359,5 +365,9 @@ def solve(I0, I1, params, u=None, p=None, dtype=np.float32):
     uo = np.empty_like(u)
     po = np.empty_like(p)
     params = np.ascontiguousarray(params, dtype=np.float64)
-    _fn("solve", dtype)(_p(params), _p(I0), _p(I1), NZ, NY, NX, _p(u), _p(p), _p(uo), _p(po))
-    return uo, po
+    if not stats:
+        _fn("solve", dtype)(_p(params), _p(I0), _p(I1), NZ, NY, NX, _p(u), _p(p), _p(uo), _p(po))
+        return uo, po
+    st = np.zeros((int(params[0]), int(params[1]), int(params[2]), 3), np.float64)
+    _fn("solve_stats", dtype)(_p(params), _p(I0), _p(I1), NZ, NY, NX, _p(u), _p(p), _p(uo), _p(po), _p(st))
+    return uo, po, st

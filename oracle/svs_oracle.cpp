@@ -672,9 +672,26 @@ template <class T> static Mesh<T> level_mesh(const SolveParams& P, int NZ, int N
   return make_mesh<T>(NZ, NY, NX, (T)(NZ - 1), (T)(NY - 1), (T)(NX - 1));
 }
 
-// One pyramid level, tvl13d.py:164-255.
+// Per-iteration statistics of one iteration n (three doubles; stats may be
+// null): the reference's residual norms tvl13d.py:222 (||p^{n+1} - p^n||),
+// :237 (||u^{n+1} - u^n||; :253's ||z - z_old|| is the same number since z = u)
+// and the energy of the iterate u^n entering the iteration,
+//   E(u^n) = lambda * sum_i |rho(u^n)_i| + lambda_tv * sum_i sum_c ||(D grad u^n)_{i,c}||_2,
+// with rho and D grad u^n exactly as the iteration evaluates them, and lambda,
+// lambda_tv the FP32 weights.  Sums and norms are taken in double.
+struct IterStats {
+  double du2 = 0, dp2 = 0, rho_abs = 0, tv = 0;
+};
+static void store_stats(const IterStats& a, double lam, double wtv, double* out) {
+  out[0] = std::sqrt(a.du2);
+  out[1] = std::sqrt(a.dp2);
+  out[2] = lam * a.rho_abs + wtv * a.tv;
+}
+
+// One pyramid level, tvl13d.py:164-255.  stats (optional) receives
+// [max_warps][max_iters][3] per-iteration statistics (see IterStats).
 template <class T> static void solve_level(const SolveParams& P, const Mesh<T>& m, const T* I0, const T* I1,
-                                           std::vector<T>& u, std::vector<T>& p) {
+                                           std::vector<T>& u, std::vector<T>& p, double* stats = nullptr) {
   const size_t N = (size_t)m.NZ * m.NY * m.NX;
   std::vector<T> G(3 * N), s(N), n(3 * N), t1(3 * N), t2(3 * N);
   nabla_fwd(I1, 1, m.NZ, m.NY, m.NX, m.hZ, m.hY, m.hX, P.bc, G.data());
@@ -708,6 +725,17 @@ template <class T> static void solve_level(const SolveParams& P, const Mesh<T>& 
       for (size_t k = 0; k < 9 * N; ++k) { const T t = round_kernel_output(sg * dz[k]); dual[k] = p[k] + t; }
       prox_dual(dual.data(), wtv, N, pn.data());
       if (P.weight_tv == 0.) std::fill(pn.begin(), pn.end(), (T)0);
+      IterStats st;
+      if (stats) {
+        for (size_t k = 0; k < 9 * N; ++k) {
+          const double d = (double)pn[k] - (double)p[k];
+          st.dp2 += d * d;
+        }
+        for (size_t r = 0; r < 3 * N; ++r) {
+          const double a = dz[3 * r], b = dz[3 * r + 1], c = dz[3 * r + 2];
+          st.tv += std::sqrt(a * a + b * b + c * c);
+        }
+      }
       p.swap(pn);
       // primal step with rho evaluated at the current u.
       for (size_t i = 0; i < N; ++i) {
@@ -721,6 +749,14 @@ template <class T> static void solve_level(const SolveParams& P, const Mesh<T>& 
       nabla_bwd(dp, 3, m.NZ, m.NY, m.NX, m.hZ, m.hY, m.hX, P.bc, div.data());
       for (size_t k = 0; k < 3 * N; ++k) { const T t = round_kernel_output(ta * div[k]); primal[k] = u[k] - t; }
       prox_primal(primal.data(), ta, lam, rho.data(), Gw.data(), N, un.data());
+      if (stats) {
+        for (size_t k = 0; k < 3 * N; ++k) {
+          const double d = (double)un[k] - (double)u[k];
+          st.du2 += d * d;
+        }
+        for (size_t i = 0; i < N; ++i) st.rho_abs += std::fabs((double)rho[i]);
+        store_stats(st, (double)lam, (double)wtv, stats + 3 * ((size_t)w * P.max_iters + it));
+      }
       u.swap(un);
     }
   }
@@ -934,8 +970,10 @@ static void grid_sample3d(const float* x, int B, int C, int IZ, int IY, int IX, 
 
 // TVL13DOpticalFlow.compute, tvl13d.py:69-162.  I0/I1 are already blurred
 // when the caller wants the blur (the blur weights come from flow_utils.py).
+// stats (optional): [num_scales][max_warps][max_iters][3] per-iteration
+// statistics, level 0 the finest (see IterStats).
 template <class T> static void solve(const SolveParams& P, const T* I0, const T* I1, int NZ, int NY, int NX,
-                                     const T* u_in, const T* p_in, T* u_out, T* p_out) {
+                                     const T* u_in, const T* p_in, T* u_out, T* p_out, double* stats = nullptr) {
   const int S = P.num_scales;
   std::vector<Mesh<T>> meshes;
   meshes.push_back(level_mesh<T>(P, NZ, NY, NX));
@@ -956,7 +994,8 @@ template <class T> static void solve(const SolveParams& P, const T* I0, const T*
     us[s].assign(3 * Ns, (T)0); ps[s].assign(9 * Ns, (T)0);
   }
   for (int s = S - 1; s >= 0; --s) {
-    solve_level(P, meshes[s], I0s[s].data(), I1s[s].data(), us[s], ps[s]);
+    solve_level(P, meshes[s], I0s[s].data(), I1s[s].data(), us[s], ps[s],
+                stats ? stats + (size_t)s * P.max_warps * P.max_iters * 3 : nullptr);
     if (s == 0) break;
     const Mesh<T>& fine = meshes[s - 1];
     const Mesh<T>& coarse = meshes[s];
@@ -1028,8 +1067,8 @@ using namespace orc;
   extern "C" void orc_median3d_##SFX(const T* in, int B, int NZ, int NY, int NX, int C, int ks, T* out) {      \
     median3d<T>(in, B, NZ, NY, NX, C, ks, out);                                                                \
   }                                                                                                            \
-  extern "C" void orc_solve_##SFX(const double* params, const T* I0, const T* I1, int NZ, int NY, int NX,       \
-                                  const T* u_in, const T* p_in, T* u_out, T* p_out) {                          \
+  extern "C" void orc_solve_stats_##SFX(const double* params, const T* I0, const T* I1, int NZ, int NY, int NX, \
+                                        const T* u_in, const T* p_in, T* u_out, T* p_out, double* stats) {     \
     SolveParams P;                                                                                             \
     P.num_scales = (int)params[0]; P.max_warps = (int)params[1]; P.max_iters = (int)params[2];                 \
     P.weight_matching = params[3]; P.weight_tv = params[4]; P.pd_type = (int)params[5];                        \
@@ -1037,8 +1076,13 @@ using namespace orc;
     P.use_aniso = (int)params[10]; P.alpha = params[11]; P.beta = params[12];                                  \
     P.ip = (int)params[13]; P.bc = (int)params[14]; P.length_fixed = (int)params[15];                          \
     P.LZ = params[16]; P.LY = params[17]; P.LX = params[18];                                                   \
-    solve<T>(P, I0, I1, NZ, NY, NX, u_in, p_in, u_out, p_out);                                                 \
+    solve<T>(P, I0, I1, NZ, NY, NX, u_in, p_in, u_out, p_out, stats);                                          \
+  }                                                                                                            \
+  extern "C" void orc_solve_##SFX(const double* params, const T* I0, const T* I1, int NZ, int NY, int NX,       \
+                                  const T* u_in, const T* p_in, T* u_out, T* p_out) {                          \
+    orc_solve_stats_##SFX(params, I0, I1, NZ, NY, NX, u_in, p_in, u_out, p_out, nullptr);                      \
   }
+
 
 ORC_INSTANTIATE(float, f32)
 ORC_INSTANTIATE(double, f64)
