@@ -41,6 +41,7 @@ def parse():
                     help="pairs timed through the reference's step-by-step driver over the drop-in operators")
     ap.add_argument("--sweep-launches", type=int, default=100, help="sweeps per level timed for the roofline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-stats", action="store_true", help="skip the per-iteration statistics leg")
     ap.add_argument("--sweep-shapes", default="",
                     help="fixed fused-sweep shapes 'bx,by,cz;...' per level instead of autotuning "
                          "(reproduces a measured configuration, e.g. under a profiler)")
@@ -293,6 +294,28 @@ def main():
     vox_it = cfg.voxel_iterations(shape)
     log(args, f"batched: {ms_per_step:.1f} ms/step, {solves_per_s:.1f} solves/s")
 
+    # The same steps with per-iteration statistics on (the residual norms the
+    # reference computes each iteration, and the energy): their cost, and the
+    # finest level's values at the first and last iteration of its last warp.
+    stats = None
+    if not args.no_stats:
+        plan.set_stats(True)
+        step()  # captures the graph again, with the statistics launches
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        st_ms = 1e3 * (time.perf_counter() - t0) / max(args.steps, 1)
+        last_warp = plan.stats[0, -1].double()  # [iterations, instances, 3]
+        stats = {"ms_per_step": round(st_ms, 3), "overhead": round(st_ms / ms_per_step - 1.0, 4)}
+        for name, it in (("first", 0), ("last", -1)):
+            v = last_warp[it].mean(0)
+            stats[f"finest_last_warp_{name}_iteration_mean"] = {
+                "primal_residual": float(v[0]), "dual_residual": float(v[1]), "energy": float(v[2])}
+        plan.set_stats(False)
+        log(args, f"with per-iteration statistics: {st_ms:.1f} ms/step ({100 * stats['overhead']:+.1f} %)")
+
     # Fused sweep kernel on its own: event-timed launches per pyramid level on
     # the stream the kernel runs on (torch's current stream).
     levels = cfg.level_shapes(shape)
@@ -470,6 +493,7 @@ def main():
                      "us_per_launch_by_level": [round(v * 1e3, 2) for v in sweep_ms]},
         "cpu_baseline": cpu,
         "accuracy": accuracy,
+        "iteration_stats": stats,
     }
     if args.serve_requests > 0:
         out["served"] = serve_leg(args, dev)

@@ -284,6 +284,28 @@ int svs_tvl1_plan_set_shape(svs_plan* plan, int level, int bx, int by, int cz, v
  * qualifies).  Results do not depend on the mode.  No reference counterpart. */
 int svs_tvl1_plan_set_resident(svs_plan* plan, int level, int mode, void* stream);
 
+/* Per-iteration statistics, the reference's residual norms made usable:
+ * tvl13d.py:222, 237 and 253 compute ||p^{n+1} - p^n||, ||u^{n+1} - u^n|| and
+ * ||z^{n+1} - z^n|| (the same number as the second, since z = u) every
+ * iteration with a host sync each and then discard them.  With out != NULL,
+ * every later run also writes, for pyramid level s (0 = finest), warp w,
+ * iteration n and instance k, three floats at
+ * out[(((s * max_warps + w) * max_outer_iterations + n) * ninst + k) * 3 + j]:
+ *   j = 0  ||u^{n+1} - u^n||_2 over the instance's voxels,
+ *   j = 1  ||p^{n+1} - p^n||_2,
+ *   j = 2  the energy of the iterate entering the iteration,
+ *          E(u^n) = lambda sum_i |rho(u^n)_i| + lambda_tv sum_i sum_c ||(D grad u^n)_{i,c}||_2
+ *          with rho and D grad u^n as the iteration evaluates them.
+ * The sweeps add the terms per work-group and wave while they update u and p
+ * (the same u and p bits as without), a reduction after each warp sums them in
+ * a fixed order in double, so the values are the same on every run.  out holds
+ * num_scales * max_warps * max_outer_iterations * ninst * 3 floats and must
+ * stay allocated while statistics are on; out = NULL turns them off.  The plan
+ * allocates its own buffer for the partial sums (freed when statistics are
+ * turned off or the plan is destroyed) and re-captures its graph at the next
+ * run.  No reference counterpart beyond the discarded norms. */
+int svs_tvl1_plan_set_stats(svs_plan* plan, float* out, void* stream);
+
 /* One frame pair without a graph: TVL13DOpticalFlow.compute(I0, I1, u, p).
  * Builds a plan in `workspace`, runs it and frees it, so it returns after the
  * solve has completed; the cached-plan calls above are the asynchronous form. */

@@ -75,6 +75,7 @@ _SIGNATURES = {
     "svs_tvl1_plan_sweeps": ([_vp, _i, _i, _vp], _i),
     "svs_tvl1_plan_set_shape": ([_vp, _i, _i, _i, _i, _vp], _i),
     "svs_tvl1_plan_set_resident": ([_vp, _i, _i, _vp], _i),
+    "svs_tvl1_plan_set_stats": ([_vp, _vp, _vp], _i),
     "svs_tvl1_plan_autotune": ([_vp, _i, _vp, ctypes.POINTER(_i)], _i),
     "svs_tvl1_solve": ([ctypes.POINTER(TVL1Params), _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t,
                         _vp], _i),

@@ -399,10 +399,15 @@ struct alignas(16) SweepTile {
 // UNIT_H: every instance of the launch has h = 1 on all three axes (always
 // so for numDofs meshes, h = (N-1)/(N-1)).  The divisions by h then drop out
 // at compile time; x / 1.0f == x, so the results are the same bits.
-template <int BC, bool ANISO, bool UNIT_H>
+// STATS: the work-group also sums, over the voxels it updates, the four
+// per-iteration statistics terms (squared change of u, squared change of p,
+// |rho(u^n)|, and the l2 norms of the rows of D grad u^n) and writes them to
+// part[blockIdx.x] (see k_stats_reduce); u and p are computed as without.
+template <int BC, bool ANISO, bool UNIT_H, bool STATS>
 __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ descs,
                                                 const SweepTile* __restrict__ tiles, int parity, float sigma,
-                                                float tau, float factor, float wtv, int wtv_zero) {
+                                                float tau, float factor, float wtv, int wtv_zero,
+                                                float4* __restrict__ part) {
   extern __shared__ float lds[];
   const SweepTile t = tiles[blockIdx.x];
   const LevelDesc& d = descs[t.inst];
@@ -486,6 +491,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   }
   for (int c = 0; c < 3; ++c) ringc[c] = bld(uin, ring_vo, c * F4 + kstart * P4);
   float r2a[3] = {0.0f, 0.0f, 0.0f}, r2b[3] = {0.0f, 0.0f, 0.0f}, r2c[3];
+  float st_du = 0.0f, st_dp = 0.0f, st_rho = 0.0f, st_tv = 0.0f;  // STATS only
   // Each step reads this thread's u of the previous plane back from its own
   // slot of the other LDS buffer (only this thread writes that slot), so no
   // register carries it across a step; seed that slot for the first step.
@@ -549,6 +555,13 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
         }
         prox_dual_row(pv, wtv);
         if (wtv_zero) { pv[0] = 0.0f; pv[1] = 0.0f; pv[2] = 0.0f; }
+        if (STATS && store_p) {
+          st_tv += __builtin_amdgcn_sqrtf(dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2]);
+          for (int e = 0; e < 3; ++e) {
+            const float dd = pv[e] - pc[3 * c + e];
+            st_dp += dd * dd;
+          }
+        }
         if (store_p)
           for (int e = 0; e < 3; ++e) bst(pout, vo, (3 * c + e) * F4 + k * P4, pv[e]);
         float rr[3];
@@ -591,6 +604,13 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
       const float sm = (um[0] * g[0] + um[2] * g[2]) + um[1] * g[1];  // (a + c) + b, as torch's sum
       const float rho = rgc[0] + sm;
       prox_primal_point(v, rho, g, factor);
+      if (STATS) {
+        st_rho += fabsf(rho);
+        for (int c = 0; c < 3; ++c) {
+          const float dd = v[c] - um[c];
+          st_du += dd * dd;
+        }
+      }
       for (int c = 0; c < 3; ++c) bst(uout, vo, c * F4 + km * P4, v[c]);
     }
     if (ANISO) {
@@ -611,6 +631,20 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
     }
     for (int e = 0; e < 9; ++e) pc[e] = pn[e];
     for (int e = 0; e < 4; ++e) { snc[e] = snn[e]; rgc[e] = rgn[e]; }
+  }
+  if constexpr (STATS) {
+    // Every thread of the work-group gets here (none returns early).
+    __shared__ float4 red[16];
+    float4 w = make_float4(wave_sum(st_du), wave_sum(st_dp), wave_sum(st_rho), wave_sum(st_tv));
+    if ((tid & 63) == 0) red[tid >> 6] = w;
+    __syncthreads();
+    if (tid == 0) {
+      const int nw = ((int)blockDim.x + 63) >> 6;
+      for (int j = 1; j < nw; ++j) {
+        w.x += red[j].x; w.y += red[j].y; w.z += red[j].z; w.w += red[j].w;
+      }
+      part[blockIdx.x] = w;
+    }
   }
 }
 
@@ -638,10 +672,14 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
 // without spills.
 constexpr int kResidentPerThread = 2;
 
+// part (null: off): per iteration it and wave, the four statistics sums of
+// the wave's voxels at part[(it * gridDim.x + blockIdx.x) * waves + wave] (see
+// k_sweep's STATS and k_stats_reduce).
 template <int BC, bool ANISO, bool UNIT_H, int V>
 __global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __restrict__ descs,
                                                          const float* __restrict__ sched, int first, int K,
-                                                         int sched_len, int parity, float wtv, int wtv_zero) {
+                                                         int sched_len, int parity, float wtv, int wtv_zero,
+                                                         float4* __restrict__ part) {
   extern __shared__ float lds[];
   const LevelDesc& d = descs[blockIdx.x];
   const int NZ = d.NZ, NY = d.NY, NX = d.NX;
@@ -680,6 +718,7 @@ __global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __rest
   for (int it = 0; it < K; ++it) {
     const int n = (first + it) % sched_len;
     const float sigma = sched[3 * n], tau = sched[3 * n + 1], factor = sched[3 * n + 2];
+    float st_du = 0.0f, st_dp = 0.0f, st_rho = 0.0f, st_tv = 0.0f;
 #pragma unroll
     for (int j = 0; j < V; ++j)
       if (own[j]) {
@@ -728,6 +767,14 @@ __global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __rest
           }
           prox_dual_row(pv, wtv);
           if (wtv_zero) { pv[0] = 0.0f; pv[1] = 0.0f; pv[2] = 0.0f; }
+          if (part) {
+            st_tv += __builtin_amdgcn_sqrtf(dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2]);
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+              const float dd = pv[e] - p[j][3 * c + e];
+              st_dp += dd * dd;
+            }
+          }
 #pragma unroll
           for (int e = 0; e < 3; ++e) p[j][3 * c + e] = pv[e];
           float rr[3];
@@ -771,9 +818,23 @@ __global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __rest
         const float sm = (u[j][0] * g[0] + u[j][2] * g[2]) + u[j][1] * g[1];  // (a + c) + b, as torch's sum
         const float rho = rgv[0] + sm;
         prox_primal_point(nv, rho, g, factor);
+        if (part) {
+          st_rho += fabsf(rho);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const float dd = nv[c] - u[j][c];
+            st_du += dd * dd;
+          }
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) u[j][c] = nv[c];
       }
+    if (part) {  // uniform: every thread of the work-group takes it
+      const float4 w = make_float4(wave_sum(st_du), wave_sum(st_dp), wave_sum(st_rho), wave_sum(st_tv));
+      const int waves = (int)blockDim.x >> 6;
+      if ((threadIdx.x & 63) == 0)
+        part[((int64_t)it * gridDim.x + blockIdx.x) * waves + (threadIdx.x >> 6)] = w;
+    }
   }
   float* uout = d.u[parity ^ (K & 1)];
   float* pout = d.p[parity ^ (K & 1)];
@@ -786,6 +847,38 @@ __global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __rest
 #pragma unroll
       for (int e = 0; e < 9; ++e) pout[(int64_t)e * N + v] = p[j][e];
     }
+}
+
+// --------------------------------------------------- iteration statistics
+// Per-iteration statistics of every instance from the sweeps' partial sums,
+// part[(n * ngroups + g) * wpg + w] = (sum (u' - u)^2, sum (p' - p)^2,
+// sum |rho(u)|, sum_c ||(D grad u)_c||) of work-group g's wave w (wpg = 1 for
+// the marching sweep, which sums its waves itself) at iteration n of a warp.
+// Work-group (n, k) of K * ninst adds instance k's entries in a fixed order, in
+// double, and writes out[(n * ninst + k) * 3 + j]: j = 0 ||u^{n+1} - u^n||,
+// j = 1 ||p^{n+1} - p^n|| (the residual norms of tvl13d.py:222, 237, 253),
+// j = 2 the energy lambda sum|rho(u^n)| + lambda_tv sum_c ||D grad u^n_c||.
+// Instance k's work-groups are csr_idx[csr_off[k] .. csr_off[k + 1]), or work-
+// group k alone when csr_off is null (the resident sweep).
+__global__ __launch_bounds__(64) void k_stats_reduce(const float4* __restrict__ part, const int* __restrict__ csr_off,
+                                                     const int* __restrict__ csr_idx, int ngroups, int wpg,
+                                                     int ninst, double lam, double wtv, float* __restrict__ out) {
+  const int n = (int)blockIdx.x / ninst, k = (int)blockIdx.x % ninst;
+  const int g0 = csr_off ? csr_off[k] : k, g1 = csr_off ? csr_off[k + 1] : k + 1;
+  const int units = (g1 - g0) * wpg;
+  double a = 0.0, b = 0.0, c = 0.0, e = 0.0;
+  for (int q = (int)threadIdx.x; q < units; q += 64) {
+    const int g = csr_off ? csr_idx[g0 + q / wpg] : k;
+    const float4 v = part[((int64_t)n * ngroups + g) * wpg + q % wpg];
+    a += v.x; b += v.y; c += v.z; e += v.w;
+  }
+  a = wave_sum(a); b = wave_sum(b); c = wave_sum(c); e = wave_sum(e);
+  if (threadIdx.x == 0) {
+    float* o = out + ((int64_t)n * ninst + k) * 3;
+    o[0] = (float)sqrt(a);
+    o[1] = (float)sqrt(b);
+    o[2] = (float)(lam * c + wtv * e);
+  }
 }
 
 // ============================================================ host plan
@@ -801,6 +894,7 @@ struct LevelPlan {
   int* py_start = nullptr;
   int px_blocks = 0, py_blocks = 0;
   SweepTile* tiles = nullptr;
+  int* tile_csr = nullptr;       // per instance, its tiles' launch indices: [ninst + 1] offsets, then indices
   int ntiles = 0;
   int tile_capacity = 0;
   int bx = 0, by = 0, cz = 0;    // shape policy (see inst_tile)
@@ -845,6 +939,11 @@ struct svs_plan {
   hipEvent_t staged[kStagingSlots] = {};
   int next_slot = 0;
   float* sched_dev = nullptr;            // (sigma, tau, tau * lambda) per iteration
+  // Per-iteration statistics (svs_tvl1_plan_set_stats): the caller's output
+  // and the plan's own buffer of the sweeps' partial sums for one warp.
+  float* stats_out = nullptr;
+  float4* stats_part = nullptr;
+  size_t stats_part_bytes = 0;
 };
 
 namespace svs {
@@ -1091,6 +1190,7 @@ static int plan_layout(svs_plan* pl, char* base, size_t* total_out) {
     }
     L.tile_capacity = cap;
     L.tiles = (SweepTile*)take(sizeof(SweepTile) * cap);
+    L.tile_csr = (int*)take(sizeof(int) * ((size_t)pl->ninst + 1 + cap));
     int nb = 0;
     bool unit = true;
     for (auto& d : L.host) {
@@ -1230,7 +1330,14 @@ static int mark_last_work(svs_plan* pl, hipStream_t st) {
   return 0;
 }
 
-// Releases the plan's host-side resources once its last work has completed.
+// Forgets the captured schedule (launch shapes or buffers changed); the next
+// graph run captures it again.
+static void drop_graph(svs_plan* pl) {
+  if (pl->exec) { (void)hipGraphExecDestroy(pl->exec); pl->exec = nullptr; }
+  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+}
+
+// Releases the plan's resources once its last work has completed.
 static void free_plan(svs_plan* pl) {
   if (pl->last_work) (void)hipEventSynchronize(pl->last_work);
   for (int i = 0; i < svs_plan::kStagingSlots; ++i)
@@ -1239,6 +1346,7 @@ static void free_plan(svs_plan* pl) {
       (void)hipEventDestroy(pl->staged[i]);
     }
   if (pl->staging) (void)hipHostFree(pl->staging);
+  if (pl->stats_part) (void)hipFree(pl->stats_part);
   if (pl->last_work) (void)hipEventDestroy(pl->last_work);
   if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
   if (pl->graph) (void)hipGraphDestroy(pl->graph);
@@ -1340,72 +1448,92 @@ static int upload_tiles(svs_plan* pl, int s, hipStream_t st) {
           if (start[g] + slot < start[g + 1]) tiles.push_back(natural[start[g] + slot]);
     }
     SVS_HIP(hipMemcpyAsync(L.tiles, tiles.data(), sizeof(SweepTile) * tiles.size(), hipMemcpyHostToDevice, st));
+    // Each instance's tiles in launch order, for the statistics reduction.
+    std::vector<int> csr((size_t)pl->ninst + 1 + tiles.size(), 0);
+    for (const SweepTile& t : tiles) csr[(size_t)t.inst + 1]++;
+    for (int k = 0; k < pl->ninst; ++k) csr[(size_t)k + 1] += csr[k];
+    {
+      std::vector<int> fill(csr.begin(), csr.begin() + pl->ninst);
+      for (int b = 0; b < (int)tiles.size(); ++b) csr[(size_t)pl->ninst + 1 + fill[tiles[b].inst]++] = b;
+    }
+    SVS_HIP(hipMemcpyAsync(L.tile_csr, csr.data(), sizeof(int) * csr.size(), hipMemcpyHostToDevice, st));
     SVS_HIP(hipStreamSynchronize(st));
   }
   return 0;
 }
 
-template <int BC, bool ANISO>
-static void launch_sweep_t(const LevelPlan& L, int parity, float sg, float ta, float fac, float wtv, int wz,
-                           hipStream_t st) {
+template <int BC, bool ANISO, bool STATS>
+static void launch_sweep_s(const LevelPlan& L, int parity, float sg, float ta, float fac, float wtv, int wz,
+                           float4* part, hipStream_t st) {
   if (L.unit_h)
-    hipLaunchKernelGGL((k_sweep<BC, ANISO, true>), dim3(L.ntiles), dim3(L.threads), L.lds, st, L.dev, L.tiles,
-                       parity, sg, ta, fac, wtv, wz);
+    hipLaunchKernelGGL((k_sweep<BC, ANISO, true, STATS>), dim3(L.ntiles), dim3(L.threads), L.lds, st, L.dev,
+                       L.tiles, parity, sg, ta, fac, wtv, wz, part);
   else
-    hipLaunchKernelGGL((k_sweep<BC, ANISO, false>), dim3(L.ntiles), dim3(L.threads), L.lds, st, L.dev, L.tiles,
-                       parity, sg, ta, fac, wtv, wz);
+    hipLaunchKernelGGL((k_sweep<BC, ANISO, false, STATS>), dim3(L.ntiles), dim3(L.threads), L.lds, st, L.dev,
+                       L.tiles, parity, sg, ta, fac, wtv, wz, part);
 }
 
-static void launch_sweep(const svs_plan* pl, const LevelPlan& L, int parity, int n, hipStream_t st) {
+template <int BC, bool ANISO>
+static void launch_sweep_t(const LevelPlan& L, int parity, float sg, float ta, float fac, float wtv, int wz,
+                           float4* part, hipStream_t st) {
+  if (part) launch_sweep_s<BC, ANISO, true>(L, parity, sg, ta, fac, wtv, wz, part, st);
+  else launch_sweep_s<BC, ANISO, false>(L, parity, sg, ta, fac, wtv, wz, nullptr, st);
+}
+
+// One sweep at schedule position n; part (null: no statistics) receives one
+// float4 of statistics sums per work-group.
+static void launch_sweep(const svs_plan* pl, const LevelPlan& L, int parity, int n, hipStream_t st,
+                         float4* part = nullptr) {
   const svs_tvl1_params& P = pl->P;
   const float sg = pl->sig[n], ta = pl->tau[n], fac = pl->factor[n];
   const float wtv = P.weight_tv;
   const int wz = P.weight_tv == 0.0f;
   const bool an = P.use_aniso != 0;
   switch (P.boundary * 2 + (an ? 1 : 0)) {
-    case 0: launch_sweep_t<BD_NEAREST, false>(L, parity, sg, ta, fac, wtv, wz, st); break;
-    case 1: launch_sweep_t<BD_NEAREST, true>(L, parity, sg, ta, fac, wtv, wz, st); break;
-    case 2: launch_sweep_t<BD_MIRROR, false>(L, parity, sg, ta, fac, wtv, wz, st); break;
-    case 3: launch_sweep_t<BD_MIRROR, true>(L, parity, sg, ta, fac, wtv, wz, st); break;
-    case 4: launch_sweep_t<BD_REFLECT, false>(L, parity, sg, ta, fac, wtv, wz, st); break;
-    default: launch_sweep_t<BD_REFLECT, true>(L, parity, sg, ta, fac, wtv, wz, st); break;
+    case 0: launch_sweep_t<BD_NEAREST, false>(L, parity, sg, ta, fac, wtv, wz, part, st); break;
+    case 1: launch_sweep_t<BD_NEAREST, true>(L, parity, sg, ta, fac, wtv, wz, part, st); break;
+    case 2: launch_sweep_t<BD_MIRROR, false>(L, parity, sg, ta, fac, wtv, wz, part, st); break;
+    case 3: launch_sweep_t<BD_MIRROR, true>(L, parity, sg, ta, fac, wtv, wz, part, st); break;
+    case 4: launch_sweep_t<BD_REFLECT, false>(L, parity, sg, ta, fac, wtv, wz, part, st); break;
+    default: launch_sweep_t<BD_REFLECT, true>(L, parity, sg, ta, fac, wtv, wz, part, st); break;
   }
 }
 
 template <int BC, bool ANISO, int V>
 static void launch_resident_v(const svs_plan* pl, const LevelPlan& L, int parity, int first, int K,
-                              hipStream_t st) {
+                              float4* part, hipStream_t st) {
   const svs_tvl1_params& P = pl->P;
   const float wtv = P.weight_tv;
   const int wz = P.weight_tv == 0.0f;
   const int len = P.max_outer_iterations;
   if (L.unit_h)
     hipLaunchKernelGGL((k_sweep_resident<BC, ANISO, true, V>), dim3(pl->ninst), dim3(L.res_threads), L.res_lds, st,
-                       L.dev, pl->sched_dev, first, K, len, parity, wtv, wz);
+                       L.dev, pl->sched_dev, first, K, len, parity, wtv, wz, part);
   else
     hipLaunchKernelGGL((k_sweep_resident<BC, ANISO, false, V>), dim3(pl->ninst), dim3(L.res_threads), L.res_lds,
-                       st, L.dev, pl->sched_dev, first, K, len, parity, wtv, wz);
+                       st, L.dev, pl->sched_dev, first, K, len, parity, wtv, wz, part);
 }
 
 template <int BC, bool ANISO>
 static void launch_resident_t(const svs_plan* pl, const LevelPlan& L, int parity, int first, int K,
-                              hipStream_t st) {
-  if (L.resident == 1) launch_resident_v<BC, ANISO, 1>(pl, L, parity, first, K, st);
-  else launch_resident_v<BC, ANISO, 2>(pl, L, parity, first, K, st);
+                              float4* part, hipStream_t st) {
+  if (L.resident == 1) launch_resident_v<BC, ANISO, 1>(pl, L, parity, first, K, part, st);
+  else launch_resident_v<BC, ANISO, 2>(pl, L, parity, first, K, part, st);
 }
 
 // K iterations of schedule positions first, first + 1, ... (cyclic) in one
-// launch; the state moves from buffer `parity` to parity ^ (K & 1).
+// launch; the state moves from buffer `parity` to parity ^ (K & 1).  part
+// (null: no statistics) receives K * ninst * (res_threads / 64) float4s.
 static void launch_resident(const svs_plan* pl, const LevelPlan& L, int parity, int first, int K,
-                            hipStream_t st) {
+                            hipStream_t st, float4* part = nullptr) {
   const bool an = pl->P.use_aniso != 0;
   switch (pl->P.boundary * 2 + (an ? 1 : 0)) {
-    case 0: launch_resident_t<BD_NEAREST, false>(pl, L, parity, first, K, st); break;
-    case 1: launch_resident_t<BD_NEAREST, true>(pl, L, parity, first, K, st); break;
-    case 2: launch_resident_t<BD_MIRROR, false>(pl, L, parity, first, K, st); break;
-    case 3: launch_resident_t<BD_MIRROR, true>(pl, L, parity, first, K, st); break;
-    case 4: launch_resident_t<BD_REFLECT, false>(pl, L, parity, first, K, st); break;
-    default: launch_resident_t<BD_REFLECT, true>(pl, L, parity, first, K, st); break;
+    case 0: launch_resident_t<BD_NEAREST, false>(pl, L, parity, first, K, part, st); break;
+    case 1: launch_resident_t<BD_NEAREST, true>(pl, L, parity, first, K, part, st); break;
+    case 2: launch_resident_t<BD_MIRROR, false>(pl, L, parity, first, K, part, st); break;
+    case 3: launch_resident_t<BD_MIRROR, true>(pl, L, parity, first, K, part, st); break;
+    case 4: launch_resident_t<BD_REFLECT, false>(pl, L, parity, first, K, part, st); break;
+    default: launch_resident_t<BD_REFLECT, true>(pl, L, parity, first, K, part, st); break;
   }
 }
 
@@ -1419,19 +1547,26 @@ static int allow_resident_lds() {
   return 0;
 }
 
+template <bool STATS>
+static int allow_sweep_lds(int v) {
+  const void* kernels[] = {
+      (const void*)k_sweep<BD_NEAREST, false, false, STATS>, (const void*)k_sweep<BD_NEAREST, true, false, STATS>,
+      (const void*)k_sweep<BD_MIRROR, false, false, STATS>,  (const void*)k_sweep<BD_MIRROR, true, false, STATS>,
+      (const void*)k_sweep<BD_REFLECT, false, false, STATS>, (const void*)k_sweep<BD_REFLECT, true, false, STATS>,
+      (const void*)k_sweep<BD_NEAREST, false, true, STATS>,  (const void*)k_sweep<BD_NEAREST, true, true, STATS>,
+      (const void*)k_sweep<BD_MIRROR, false, true, STATS>,   (const void*)k_sweep<BD_MIRROR, true, true, STATS>,
+      (const void*)k_sweep<BD_REFLECT, false, true, STATS>,  (const void*)k_sweep<BD_REFLECT, true, true, STATS>};
+  for (const void* k : kernels) SVS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, v));
+  return 0;
+}
+
 // Dynamic LDS above 64 KiB must be enabled per kernel.
 static int allow_large_lds(const svs_plan* pl) {
   // shape_ok keeps every tile within 96 KiB; enable that much once.
   (void)pl;
   const int v = 96 * 1024;
-  const void* kernels[] = {
-      (const void*)k_sweep<BD_NEAREST, false, false>, (const void*)k_sweep<BD_NEAREST, true, false>,
-      (const void*)k_sweep<BD_MIRROR, false, false>,  (const void*)k_sweep<BD_MIRROR, true, false>,
-      (const void*)k_sweep<BD_REFLECT, false, false>, (const void*)k_sweep<BD_REFLECT, true, false>,
-      (const void*)k_sweep<BD_NEAREST, false, true>,  (const void*)k_sweep<BD_NEAREST, true, true>,
-      (const void*)k_sweep<BD_MIRROR, false, true>,   (const void*)k_sweep<BD_MIRROR, true, true>,
-      (const void*)k_sweep<BD_REFLECT, false, true>,  (const void*)k_sweep<BD_REFLECT, true, true>};
-  for (const void* k : kernels) SVS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, v));
+  if (int rc = allow_sweep_lds<false>(v)) return rc;
+  if (int rc = allow_sweep_lds<true>(v)) return rc;
   // The resident sweep holds 48 bytes per voxel of up to 2048 voxels.
   if (int rc = allow_resident_lds<BD_NEAREST, false>()) return rc;
   if (int rc = allow_resident_lds<BD_NEAREST, true>()) return rc;
@@ -1439,6 +1574,38 @@ static int allow_large_lds(const svs_plan* pl) {
   if (int rc = allow_resident_lds<BD_MIRROR, true>()) return rc;
   if (int rc = allow_resident_lds<BD_REFLECT, false>()) return rc;
   if (int rc = allow_resident_lds<BD_REFLECT, true>()) return rc;
+  return 0;
+}
+
+// Bytes of partial sums one warp's statistics need at the current launch
+// shapes: K iterations x (one float4 per marching work-group, or per wave of
+// a resident work-group), the largest over the levels.
+static size_t stats_part_need(const svs_plan* pl) {
+  const size_t K = (size_t)std::max(1, pl->P.max_outer_iterations);
+  size_t need = 0;
+  for (const LevelPlan& L : pl->levels) {
+    need = std::max(need, K * (size_t)L.ntiles * sizeof(float4));
+    if (L.res_capable) need = std::max(need, K * (size_t)pl->ninst * (L.res_threads / 64) * sizeof(float4));
+  }
+  return need;
+}
+
+// With statistics on, (re)allocates the partial-sum buffer if the current
+// launch shapes need more than it holds (after the plan's previous work, which
+// may still read it; the captured schedule then holds the old address).
+static int ensure_stats_part(svs_plan* pl) {
+  if (!pl->stats_out) return 0;
+  const size_t need = stats_part_need(pl);
+  if (need <= pl->stats_part_bytes) return 0;
+  SVS_HIP(hipEventSynchronize(pl->last_work));
+  if (pl->stats_part) {
+    (void)hipFree(pl->stats_part);
+    pl->stats_part = nullptr;
+    pl->stats_part_bytes = 0;
+  }
+  SVS_HIP(hipMalloc((void**)&pl->stats_part, need));
+  pl->stats_part_bytes = need;
+  drop_graph(pl);
   return 0;
 }
 
@@ -1468,19 +1635,33 @@ static int enqueue_solve(svs_plan* pl, hipStream_t st) {
       SVS_LAUNCHED();
     }
     parity = 0;
+    const int K = P.max_outer_iterations;
     for (int w = 0; w < P.max_warps; ++w) {
       k_linearise<<<L.nblocks, kBlock, 0, st>>>(L.dev, L.block_start, n, parity, ip, bc);
       SVS_LAUNCHED();
-      if (L.resident && P.max_outer_iterations > 0) {
-        launch_resident(pl, L, parity, 0, P.max_outer_iterations, st);
+      // Statistics of this warp's K iterations: out[s][w][n][k][3].
+      float* sout = pl->stats_out ? pl->stats_out + ((size_t)s * P.max_warps + w) * K * n * 3 : nullptr;
+      float4* part = sout ? pl->stats_part : nullptr;
+      if (L.resident && K > 0) {
+        launch_resident(pl, L, parity, 0, K, st, part);
         SVS_LAUNCHED();
-        parity ^= P.max_outer_iterations & 1;
+        if (sout) {
+          k_stats_reduce<<<K * n, 64, 0, st>>>(part, nullptr, nullptr, n, L.res_threads / 64, n,
+                                               (double)P.weight_matching, (double)P.weight_tv, sout);
+          SVS_LAUNCHED();
+        }
+        parity ^= K & 1;
         continue;
       }
-      for (int it = 0; it < P.max_outer_iterations; ++it) {
-        launch_sweep(pl, L, parity, it, st);
+      for (int it = 0; it < K; ++it) {
+        launch_sweep(pl, L, parity, it, st, part ? part + (size_t)it * L.ntiles : nullptr);
         SVS_LAUNCHED();
         parity ^= 1;
+      }
+      if (sout && K > 0) {
+        k_stats_reduce<<<K * n, 64, 0, st>>>(part, L.tile_csr, L.tile_csr + n + 1, L.ntiles, 1, n,
+                                             (double)P.weight_matching, (double)P.weight_tv, sout);
+        SVS_LAUNCHED();
       }
     }
     if (s > 0) {
@@ -1585,6 +1766,7 @@ int svs_tvl1_plan_run(svs_plan* pl, int use_graph, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   for (int k = 0; k < pl->ninst; ++k)
     SVS_REQUIRE(pl->levels[0].host[k].src0, "instance %d has no bound input", k);
+  if (int rc = ensure_stats_part(pl)) return rc;
   if (int rc = after_last_work(pl, st)) return rc;
   if (pl->tables_dirty)
     if (int rc = upload_tables(pl, st)) return rc;
@@ -1760,8 +1942,7 @@ int svs_tvl1_plan_autotune(svs_plan* pl, int reps, void* stream, int* report /* 
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (pl->exec) { (void)hipGraphExecDestroy(pl->exec); pl->exec = nullptr; }
-  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  drop_graph(pl);
   if (!rc) rc = mark_last_work(pl, st);
   return rc;
 }
@@ -1777,8 +1958,25 @@ int svs_tvl1_plan_set_shape(svs_plan* pl, int level, int bx, int by, int cz, voi
   if (int rc = after_last_work(pl, st)) return rc;
   L.bx = bx; L.by = by; L.cz = cz;
   if (int rc = upload_tiles(pl, level, st)) return rc;
-  if (pl->exec) { (void)hipGraphExecDestroy(pl->exec); pl->exec = nullptr; }
-  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  drop_graph(pl);
+  return mark_last_work(pl, st);
+}
+
+int svs_tvl1_plan_set_stats(svs_plan* pl, float* out, void* stream) {
+  SVS_REQUIRE(pl, "null plan");
+  const int64_t groups = (int64_t)pl->P.max_outer_iterations * pl->ninst;
+  SVS_REQUIRE(groups < ((int64_t)1 << 31), "max_outer_iterations x instances too large for statistics");
+  hipStream_t st = (hipStream_t)stream;
+  if (int rc = after_last_work(pl, st)) return rc;
+  if (out != pl->stats_out) drop_graph(pl);
+  pl->stats_out = out;
+  if (!out && pl->stats_part) {  // off: release the partial sums once no work reads them
+    SVS_HIP(hipEventSynchronize(pl->last_work));
+    (void)hipFree(pl->stats_part);
+    pl->stats_part = nullptr;
+    pl->stats_part_bytes = 0;
+  }
+  if (int rc = ensure_stats_part(pl)) return rc;
   return mark_last_work(pl, st);
 }
 
@@ -1791,8 +1989,7 @@ int svs_tvl1_plan_set_resident(svs_plan* pl, int level, int mode, void* stream) 
   hipStream_t st = (hipStream_t)stream;
   if (int rc = after_last_work(pl, st)) return rc;
   L.resident = mode == 0 ? 0 : L.res_capable;
-  if (pl->exec) { (void)hipGraphExecDestroy(pl->exec); pl->exec = nullptr; }
-  if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+  drop_graph(pl);
   return mark_last_work(pl, st);
 }
 

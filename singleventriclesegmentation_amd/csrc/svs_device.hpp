@@ -385,6 +385,20 @@ __device__ __forceinline__ void bst(svs_rsrc r, uint32_t voff, uint32_t soff, fl
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, wave_uniform(soff), 0);
 }
 
+// Sum over the 64 lanes of a wave, returned to every lane.  The butterfly
+// pairs lanes symmetrically, so every lane gets the same bits, and the order is
+// fixed, so every run does too.
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
 // Linear voxel index -> (z, y, x).
 __device__ __forceinline__ void unflatten(int64_t i, int NY, int NX, int& z, int& y, int& x) {
   x = (int)(i % NX);

@@ -131,6 +131,7 @@ class Plan:
                                          ctypes.c_size_t(workspace.numel()), stream_ptr(), ctypes.byref(h)))
         self._h = h
         self._bound = [None] * self.n
+        self.stats = None
 
     def info(self, level):
         out = (ctypes.c_int * 8)()
@@ -185,6 +186,25 @@ class Plan:
         marching sweep once per iteration, -1 the automatic choice."""
         with torch.cuda.device(self.device):
             check(_lib.lib().svs_tvl1_plan_set_resident(self._h, int(level), int(mode), stream_ptr()))
+
+    def set_stats(self, enabled=True):
+        """Per-iteration statistics for every later run (or none with False).
+
+        ``self.stats`` becomes a float32 tensor [levels, warps, iterations,
+        instances, 3] on the plan's device, level 0 the finest, which each run
+        fills in stream order: [..., 0] = ||u^{n+1} - u^n||, [..., 1] =
+        ||p^{n+1} - p^n|| (the residual norms tvl13d.py:222, 237, 253 computes
+        and discards), [..., 2] = the energy lambda sum|rho(u^n)| + lambda_tv
+        sum_c ||D grad u^n_c|| of the iterate entering iteration n.  The values
+        are the same on every run; u and p do not change."""
+        c = self.cfg
+        out = None
+        if enabled:
+            out = torch.zeros((c.num_scales, c.max_warps, c.max_outer_iterations, self.n, 3), dtype=torch.float32,
+                              device=self.device)
+        with torch.cuda.device(self.device):
+            check(_lib.lib().svs_tvl1_plan_set_stats(self._h, ptr(out), stream_ptr()))
+        self.stats = out
 
     def sweeps(self, level, count):
         """Enqueues ``count`` fused sweeps of one level (timing aid; needs a prior run)."""

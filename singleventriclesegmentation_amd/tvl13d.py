@@ -13,11 +13,17 @@ at 80^3 with the 4 x 10 x 50 schedule).
 import math
 from typing import Literal, Tuple
 
+import numpy as np
 import torch
 
 from . import opticalFlow
 from .flow_utils import apply_gaussian_blur3d, get_interpolation_type, get_mesh_length
 from .solver import Plan, SolverConfig
+
+
+def _stats_dict(t):
+    """[levels, warps, iterations, 3] -> the named statistics (views of t)."""
+    return {"primal": t[..., 0], "dual": t[..., 1], "update": t[..., 0], "energy": t[..., 2]}
 
 
 class TVL13DOpticalFlow:
@@ -41,13 +47,22 @@ class TVL13DOpticalFlow:
         device: torch.device,
         use_graph: bool = True,
         autotune: bool = False,
+        record_stats: bool = False,
     ):
         """Arguments as in tvl13d.py:14-52; ``length`` is (lx, ly, lz).
 
         Additive options: ``use_graph`` replays the fixed level x warp x
         iteration schedule as a hipGraph; ``autotune`` times the fused-sweep
         launch shapes after the first solve of each volume size and keeps the
-        fastest (results do not depend on the shape)."""
+        fastest (results do not depend on the shape); ``record_stats`` makes
+        every compute (and compute_unfused) leave per-iteration statistics in
+        ``self.stats``: a dict of float32 device tensors [num_scales, max_warps,
+        max_outer_iterations] (level 0 the finest) holding the residual norms
+        the reference computes and discards each iteration, "primal" =
+        ||u^{n+1} - u^n|| (tvl13d.py:237; also its ||z - z_old||, :253, since
+        z = u) and "dual" = ||p^{n+1} - p^n|| (:222), and "energy" = lambda
+        sum|rho(u^n)| + lambda_tv sum_c ||D grad u^n_c|| of the iterate
+        entering each iteration."""
         self.num_scales = num_scales
         self.max_warps = max_warps
         self.max_outer_iterations = max_outer_iterations
@@ -67,6 +82,8 @@ class TVL13DOpticalFlow:
         self.device = device
         self.use_graph = use_graph
         self.autotune = autotune
+        self.record_stats = record_stats
+        self.stats = None
         self.config = SolverConfig(
             num_scales=num_scales, max_warps=max_warps, max_outer_iterations=max_outer_iterations,
             weight_matching=weight_matching, weight_tv=weight_tv, pd_type=primaldual_algorithm_type,
@@ -107,10 +124,14 @@ class TVL13DOpticalFlow:
             u_in = u.to(device=I0.device, dtype=torch.float32).contiguous()
             p_in = p.to(device=I0.device, dtype=torch.float32).contiguous()
         plan = self._plan(shape, I0.device, initial)
+        if self.record_stats != (plan.stats is not None):
+            plan.set_stats(self.record_stats)
         u_out = torch.empty(shape + (3,), dtype=torch.float32, device=I0.device)
         p_out = torch.empty(shape + (3, 3), dtype=torch.float32, device=I0.device)
         plan.bind(0, I0, I1, u_out=u_out, p_out=p_out, u_in=u_in, p_in=p_in)
         plan.run(use_graph=self.use_graph)
+        if self.record_stats:
+            self.stats = _stats_dict(plan.stats[:, :, :, 0].clone())
         if self.autotune and not getattr(plan, "tuned", False):
             plan.autotune()
             plan.tuned = True
@@ -143,6 +164,11 @@ class TVL13DOpticalFlow:
     def compute_unfused(self, I0, I1, u, p):
         """tvl13d.py:122-162 step by step (one operator call per reference call)."""
         I0s, I1s, us, ps, meshes = self.generate_pyramid(I0, I1, u, p)
+        rec = None
+        if self.record_stats:
+            rec = torch.zeros((self.num_scales, self.max_warps, self.max_outer_iterations, 3), dtype=torch.float32,
+                              device=I0.device)
+        self._rec = rec
         for s in range(self.num_scales - 1, -1, -1):
             us[s], ps[s] = self.step(s, I0s[s], I1s[s], us[s], ps[s], meshes[s])
             if s == 0:
@@ -153,11 +179,16 @@ class TVL13DOpticalFlow:
             us[s - 1][..., 1] *= meshes[s - 1].getLY() / meshes[s].getLY()
             us[s - 1][..., 2] *= meshes[s - 1].getLZ() / meshes[s].getLZ()
             ps[s - 1] = op.forwardMatrixField(ps[s])
+        if rec is not None:
+            self.stats = _stats_dict(rec)
         return us[0], ps[0]
 
     def step(self, s, I0, I1, u, p, meshInfo):
         """One pyramid level (tvl13d.py:164-255) without the residual .item() syncs,
-        whose values the reference never reads."""
+        whose values the reference never reads; with ``record_stats`` the same
+        norms (torch.norm of the differences, as tvl13d.py:222, 237 compute
+        them) and the energy stay on the device in ``self.stats``."""
+        rec = getattr(self, "_rec", None)
         nabla = opticalFlow.Nabla3D_CD(meshInfo, self.boundary_type)
         warping = opticalFlow.Warping3D(meshInfo, self.interpolation_type, self.boundary_type)
         I1_grad = nabla.forward(I1)
@@ -166,12 +197,13 @@ class TVL13DOpticalFlow:
             aniso = opticalFlow.AnisotropicNabla3D(meshInfo, self.anisotropic_diff_alpha, self.anistropic_diff_beta)
             tangents = aniso.computeTangentVecs(I1_grad)
         z = u
-        for _ in range(self.max_warps):
+        for w in range(self.max_warps):
             I1_warped = warping.forward(I1, u)
             I1_warped_grad = warping.forwardVectorField(I1_grad, u)
             rho_c = I1_warped - torch.sum(u * I1_warped_grad, dim=3) - I0
             sigma, tau, gamma = self.sigma, self.tau, self.gamma
-            for _ in range(self.max_outer_iterations):
+            for n in range(self.max_outer_iterations):
+                pold, uold = p, u
                 z_grad = nabla.forwardVectorField(z)
                 Dz_grad = aniso.forwardVectorField(z_grad, *tangents) if aniso else z_grad
                 p = opticalFlow.TVL1OF3D_proxDual(p + sigma * Dz_grad, sigma, self.weight_tv, meshInfo)
@@ -182,6 +214,12 @@ class TVL13DOpticalFlow:
                 Dp_div = nabla.backwardVectorField(Dp)
                 u = opticalFlow.TVL1OF3D_proxPrimal(u - tau * Dp_div, tau, self.weight_matching, rho,
                                                     I1_warped_grad, meshInfo)
+                if rec is not None:
+                    rec[s, w, n, 0] = torch.norm(u - uold)
+                    rec[s, w, n, 1] = torch.norm(p - pold)
+                    rec[s, w, n, 2] = (float(np.float32(self.weight_matching)) * rho.abs().sum()
+                                       + float(np.float32(self.weight_tv))
+                                       * torch.linalg.vector_norm(Dz_grad, dim=-1).sum())
                 if self.primaldual_algorithm_type == 2:
                     theta = 1. / math.sqrt(1. + 2. * gamma * tau)
                     tau *= theta

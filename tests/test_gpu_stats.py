@@ -1,0 +1,122 @@
+"""Per-iteration statistics of the fused solve (the reference's residual norms
+tvl13d.py:222, 237, 253 and the energy of each iterate): against the oracle's,
+against the reference-shaped driver's torch.norm values, across batch
+layouts and launch modes, and without effect on u and p."""
+import numpy as np
+import pytest
+import torch
+
+from singleventriclesegmentation_amd.flow_utils import gaussian_weights3d
+from singleventriclesegmentation_amd.solver import Plan, SolverConfig
+
+from test_gpu_solver import make_solver, pair
+
+pytestmark = pytest.mark.gpu
+
+# The fused path adds each work-group's terms in float and the work-groups in
+# double; the oracle adds everything in double: measured relative differences
+# ~6e-8 at these sizes.
+RTOL = 1e-6
+
+
+def oracle_stats(O, I0, I1, S, W, K, boundary="MIRROR", aniso=True):
+    w = gaussian_weights3d(1.0)
+    prm = O.solve_params(num_scales=S, max_warps=W, max_outer_iterations=K, boundary=boundary, use_aniso=aniso)
+    return O.solve(O.blur(I0, w), O.blur(I1, w), prm, stats=True)
+
+
+def plan_stats(dev, frames, S, W, K, boundary="MIRROR", aniso=True, resident=None, graph=True, stats=True):
+    cfg = SolverConfig(num_scales=S, max_warps=W, max_outer_iterations=K, boundary={"NEAREST": 0, "MIRROR": 1,
+                                                                                    "REFLECT": 2}[boundary],
+                       use_aniso=aniso)
+    shapes = [f[0].shape for f in frames]
+    plan = Plan(cfg, shapes, device=dev)
+    outs = []
+    for k, (a, b) in enumerate(frames):
+        I0, I1 = torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)
+        u = torch.empty(a.shape + (3,), device=dev)
+        p = torch.empty(a.shape + (3, 3), device=dev)
+        plan.bind(k, I0, I1, u_out=u, p_out=p)
+        outs.append((I0, I1, u, p))
+    if resident is not None:
+        for lv, mode in resident.items():
+            plan.set_resident(lv, mode)
+    if stats:
+        plan.set_stats(True)
+    plan.run(use_graph=graph)
+    torch.cuda.synchronize()
+    st = plan.stats.cpu().numpy().astype(np.float64) if stats else None
+    res = [(u.cpu().numpy(), p.cpu().numpy()) for (_, _, u, p) in outs]
+    plan.close()
+    return st, res
+
+
+@pytest.mark.parametrize("boundary,aniso", [("MIRROR", True), ("NEAREST", False), ("REFLECT", True)])
+def test_stats_match_oracle(O, dev, boundary, aniso):
+    """20x24x28 at two levels: level 0 runs the marching sweep, level 1
+    (10x12x14) the resident sweep with two voxels per thread."""
+    I0, I1, _ = pair((20, 24, 28))
+    S, W, K = 2, 3, 8
+    uo, po, so = oracle_stats(O, I0, I1, S, W, K, boundary, aniso)
+    st, [(u, p)] = plan_stats(dev, [(I0, I1)], S, W, K, boundary, aniso)
+    assert st.shape == (S, W, K, 1, 3)
+    assert np.array_equal(u, uo) and np.array_equal(p, po)
+    np.testing.assert_allclose(st[:, :, :, 0], so, rtol=RTOL)
+    print(f"{boundary}: max rel diff {np.max(np.abs(st[:, :, :, 0] - so) / np.maximum(so, 1e-30)):.2e}")
+
+
+def test_stats_do_not_change_the_solve_and_repeat_exactly(dev):
+    I0, I1, _ = pair((24, 20, 28), seed=3)
+    S, W, K = 3, 2, 6
+    st1, [(u1, p1)] = plan_stats(dev, [(I0, I1)], S, W, K)
+    st2, [(u2, p2)] = plan_stats(dev, [(I0, I1)], S, W, K, graph=False)
+    _, [(u0, p0)] = plan_stats(dev, [(I0, I1)], S, W, K, stats=False)
+    assert np.array_equal(u1, u0) and np.array_equal(p1, p0)
+    assert np.array_equal(u2, u0) and np.array_equal(p2, p0)
+    assert np.array_equal(st1, st2)  # fixed summation order: the same bits, graph or not
+
+
+def test_resident_and_marching_levels_give_the_same_stats(O, dev):
+    """A 10^3 level runs resident (one voxel per thread) or marching; the
+    statistics agree with each other and with the oracle."""
+    I0, I1, _ = pair((20, 20, 20), seed=5)
+    S, W, K = 2, 2, 7
+    _, _, so = oracle_stats(O, I0, I1, S, W, K)
+    sr, _ = plan_stats(dev, [(I0, I1)], S, W, K, resident={1: 1})
+    sm, _ = plan_stats(dev, [(I0, I1)], S, W, K, resident={1: 0})
+    np.testing.assert_allclose(sr[:, :, :, 0], so, rtol=RTOL)
+    np.testing.assert_allclose(sm[:, :, :, 0], so, rtol=RTOL)
+    np.testing.assert_allclose(sr, sm, rtol=RTOL)
+
+
+def test_uneven_batch_stats_per_instance(O, dev):
+    """Three sizes in one plan: each instance's statistics match its own
+    oracle solve (tiles of several instances interleave in the launches)."""
+    sizes = [(16, 18, 20), (24, 16, 12), (12, 12, 30)]
+    frames = []
+    for i, sh in enumerate(sizes):
+        a, b, _ = pair(sh, seed=10 + i)
+        frames.append((a, b))
+    S, W, K = 2, 2, 5
+    st, res = plan_stats(dev, frames, S, W, K)
+    for k, (a, b) in enumerate(frames):
+        uo, po, so = oracle_stats(O, a, b, S, W, K)
+        assert np.array_equal(res[k][0], uo) and np.array_equal(res[k][1], po)
+        np.testing.assert_allclose(st[:, :, :, k], so, rtol=RTOL)
+
+
+def test_stats_match_the_reference_driver(dev):
+    """compute (fused) against compute_unfused, which records torch.norm of the
+    differences as the reference's tvl13d.py:222, 237 compute them."""
+    I0, I1, _ = pair((32, 32, 32), seed=7)
+    a, b = torch.from_numpy(I0).to(dev), torch.from_numpy(I1).to(dev)
+    solver = make_solver(2, 3, 10, record_stats=True)
+    u_f, p_f = solver.compute(a, b, None, None)
+    fused = {k: v.cpu().numpy().astype(np.float64) for k, v in solver.stats.items()}
+    u_r, p_r = solver.compute_unfused(a, b, None, None)
+    ref = {k: v.cpu().numpy().astype(np.float64) for k, v in solver.stats.items()}
+    assert torch.equal(u_f, u_r) and torch.equal(p_f, p_r)
+    assert fused["primal"].shape == (2, 3, 10)
+    for k in ("primal", "dual", "update", "energy"):
+        np.testing.assert_allclose(fused[k], ref[k], rtol=1e-4)
+    assert np.array_equal(fused["update"], fused["primal"])
