@@ -207,6 +207,9 @@ typedef struct svs_tvl1_params {
   float LZ, LY, LX;
   int use_initial_state;      /* num_scales == 1 only: start from the bound u_in, p_in */
   int sweep_bx, sweep_by, sweep_cz; /* fused-sweep shape policy (see svs_tvl1_plan_set_shape); 0 = automatic */
+  double huber_eps;           /* >= 0; 0 is the reference's TV.  eps > 0: Huber-TV, each dual row becomes
+                                 v_c / max(1 + sigma_n eps / weight_tv, ||v_c|| / weight_tv) (no reference
+                                 counterpart: tvl13d.py has no eps) */
 } svs_tvl1_params;
 
 /* Opaque solve plan: pyramid geometry, workspace layout, device tables and an

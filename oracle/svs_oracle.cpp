@@ -591,14 +591,19 @@ template <class T> static void prox_primal(const T* primal, T tau, T lam, const 
   }
 }
 // opticalFlowOps.cu:179-208: row-wise projection onto the l2 ball of radius w.
-template <class T> static void prox_dual(const T* dual, T wtv, size_t N, T* out) {
+// hub (not in the reference, which is hub = 1): the Huber-TV form of the same
+// step, p_c = v_c / max(hub, ||v_c|| / lambda_tv) with hub = 1 + sigma eps /
+// lambda_tv, i.e. v / hub projected onto the ball (the prox of the conjugate of
+// lambda_tv H_eps(||.||)) written with one division; hub = 1 is the
+// reference's projection, bit for bit.
+template <class T> static void prox_dual(const T* dual, T wtv, size_t N, T* out, T hub = (T)1) {
 #pragma omp parallel for schedule(static)
   for (long long ii = 0; ii < (long long)N; ++ii) {
     const size_t i = (size_t)ii;
     for (int c = 0; c < 3; ++c) {
       T ns = 0;
       for (int d = 0; d < 3; ++d) ns += dual[9 * i + 3 * c + d] * dual[9 * i + 3 * c + d];
-      const T den = fmax_t((T)1, sqrt_t(ns) / wtv);
+      const T den = fmax_t(hub, sqrt_t(ns) / wtv);
       for (int d = 0; d < 3; ++d) out[9 * i + 3 * c + d] = dual[9 * i + 3 * c + d] / den;
     }
   }
@@ -650,7 +655,23 @@ struct SolveParams {
   int ip, bc;
   int length_fixed;          // 0: numDofs (L = N-1), 1: fixed lengths below
   double LZ, LY, LX;
+  double huber_eps;          // 0: the reference's TV (see prox_dual)
 };
+
+// Per-iteration Huber factor hub_n = 1 + sigma_n eps / lambda_tv, formed in
+// double from the Python-float sigma_n and the FP32 lambda_tv, used in T.
+template <class T> static T huber_factor(const SolveParams& P, double sigma_n) {
+  const T wtv = (T)P.weight_tv;
+  if (P.huber_eps == 0.0 || wtv == (T)0) return (T)1;
+  return (T)(1.0 + sigma_n * P.huber_eps / (double)wtv);
+}
+
+// The Huber function of a row norm t: t^2 / (2 eps) for t <= eps, t - eps / 2
+// beyond (eps = 0: t), evaluated in double for the energy.
+static double huber(double t, double eps) {
+  if (eps == 0.0) return t;
+  return t <= eps ? t * t / (2.0 * eps) : t - 0.5 * eps;
+}
 
 // The step-size schedule of tvl13d.py:208-246 in Python floats (double).
 static void step_schedule(const SolveParams& P, std::vector<double>& sig, std::vector<double>& tau) {
@@ -675,8 +696,9 @@ template <class T> static Mesh<T> level_mesh(const SolveParams& P, int NZ, int N
 // Per-iteration statistics of one iteration n (three doubles; stats may be
 // null): the reference's residual norms tvl13d.py:222 (||p^{n+1} - p^n||),
 // :237 (||u^{n+1} - u^n||; :253's ||z - z_old|| is the same number since z = u)
-// and the energy of the iterate u^n entering the iteration,
-//   E(u^n) = lambda * sum_i |rho(u^n)_i| + lambda_tv * sum_i sum_c ||(D grad u^n)_{i,c}||_2,
+// and the energy of the iterate u^n entering the iteration (H_eps = huber(),
+// the identity for the reference's eps = 0),
+//   E(u^n) = lambda * sum_i |rho(u^n)_i| + lambda_tv * sum_i sum_c H_eps(||(D grad u^n)_{i,c}||_2),
 // with rho and D grad u^n exactly as the iteration evaluates them, and lambda,
 // lambda_tv the FP32 weights.  Sums and norms are taken in double.
 struct IterStats {
@@ -723,7 +745,7 @@ template <class T> static void solve_level(const SolveParams& P, const Mesh<T>& 
       const T* dz = zg.data();
       if (P.use_aniso) { aniso_apply(zg.data(), s.data(), n.data(), t1.data(), t2.data(), N, Dz.data()); dz = Dz.data(); }
       for (size_t k = 0; k < 9 * N; ++k) { const T t = round_kernel_output(sg * dz[k]); dual[k] = p[k] + t; }
-      prox_dual(dual.data(), wtv, N, pn.data());
+      prox_dual(dual.data(), wtv, N, pn.data(), huber_factor<T>(P, sig[it]));
       if (P.weight_tv == 0.) std::fill(pn.begin(), pn.end(), (T)0);
       IterStats st;
       if (stats) {
@@ -733,7 +755,7 @@ template <class T> static void solve_level(const SolveParams& P, const Mesh<T>& 
         }
         for (size_t r = 0; r < 3 * N; ++r) {
           const double a = dz[3 * r], b = dz[3 * r + 1], c = dz[3 * r + 2];
-          st.tv += std::sqrt(a * a + b * b + c * c);
+          st.tv += huber(std::sqrt(a * a + b * b + c * c), P.huber_eps);
         }
       }
       p.swap(pn);
@@ -1075,7 +1097,7 @@ using namespace orc;
     P.sigma = params[6]; P.tau = params[7]; P.theta = params[8]; P.gamma = params[9];                          \
     P.use_aniso = (int)params[10]; P.alpha = params[11]; P.beta = params[12];                                  \
     P.ip = (int)params[13]; P.bc = (int)params[14]; P.length_fixed = (int)params[15];                          \
-    P.LZ = params[16]; P.LY = params[17]; P.LX = params[18];                                                   \
+    P.LZ = params[16]; P.LY = params[17]; P.LX = params[18]; P.huber_eps = params[19];                         \
     solve<T>(P, I0, I1, NZ, NY, NX, u_in, p_in, u_out, p_out, stats);                                          \
   }                                                                                                            \
   extern "C" void orc_solve_##SFX(const double* params, const T* I0, const T* I1, int NZ, int NY, int NX,       \

@@ -35,6 +35,7 @@ class TVL1Params(ctypes.Structure):
         ("length_fixed", _i), ("LZ", _f), ("LY", _f), ("LX", _f),
         ("use_initial_state", _i),
         ("sweep_bx", _i), ("sweep_by", _i), ("sweep_cz", _i),
+        ("huber_eps", ctypes.c_double),
     ]
 
 

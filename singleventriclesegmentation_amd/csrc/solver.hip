@@ -403,10 +403,13 @@ struct alignas(16) SweepTile {
 // per-iteration statistics terms (squared change of u, squared change of p,
 // |rho(u^n)|, and the l2 norms of the rows of D grad u^n) and writes them to
 // part[blockIdx.x] (see k_stats_reduce); u and p are computed as without.
-template <int BC, bool ANISO, bool UNIT_H, bool STATS>
+// HUBER: the dual rows use the Huber-TV step with factor hub (prox_dual_row)
+// and the energy the Huber function with eps; without, hub = 1 and eps = 0 are
+// compile-time constants, so the reference's step costs nothing extra.
+template <int BC, bool ANISO, bool UNIT_H, bool STATS, bool HUBER>
 __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ descs,
                                                 const SweepTile* __restrict__ tiles, int parity, float sigma,
-                                                float tau, float factor, float wtv, int wtv_zero,
+                                                float tau, float factor, float wtv, float hub, float eps,
                                                 float4* __restrict__ part) {
   extern __shared__ float lds[];
   const SweepTile t = tiles[blockIdx.x];
@@ -553,10 +556,11 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
           const float inc = sigma * dq[e];
           pv[e] = pc[3 * c + e] + inc;
         }
-        prox_dual_row(pv, wtv);
-        if (wtv_zero) { pv[0] = 0.0f; pv[1] = 0.0f; pv[2] = 0.0f; }
+        prox_dual_row(pv, wtv, HUBER ? hub : 1.0f);
+        if (wtv == 0.0f) { pv[0] = 0.0f; pv[1] = 0.0f; pv[2] = 0.0f; }  // tvl13d.py:223-224
         if (STATS && store_p) {
-          st_tv += __builtin_amdgcn_sqrtf(dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2]);
+          st_tv += huber_of_norm(__builtin_amdgcn_sqrtf(dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2]),
+                                 HUBER ? eps : 0.0f);
           for (int e = 0; e < 3; ++e) {
             const float dd = pv[e] - pc[3 * c + e];
             st_dp += dd * dd;
@@ -664,7 +668,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
 // Each voxel is updated with the same operations, in the same order, as the
 // marching sweep (k_sweep), so the results are the same bits; the launch
 // replaces K launches whose cost at these sizes is latency.
-// sched holds (sigma, tau, tau * lambda) per iteration; the state is read
+// sched holds (sigma, tau, tau * lambda, Huber factor) per iteration; the state is read
 // from buffer `parity` and left in buffer parity ^ (K & 1), where K single
 // sweeps would leave it.  With two voxels per thread, (s, n) and (rho_c, g)
 // are re-read from global memory (cache-resident after the first iteration)
@@ -679,7 +683,7 @@ template <int BC, bool ANISO, bool UNIT_H, int V>
 __global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __restrict__ descs,
                                                          const float* __restrict__ sched, int first, int K,
                                                          int sched_len, int parity, float wtv, int wtv_zero,
-                                                         float4* __restrict__ part) {
+                                                         float eps, float4* __restrict__ part) {
   extern __shared__ float lds[];
   const LevelDesc& d = descs[blockIdx.x];
   const int NZ = d.NZ, NY = d.NY, NX = d.NX;
@@ -717,7 +721,7 @@ __global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __rest
 
   for (int it = 0; it < K; ++it) {
     const int n = (first + it) % sched_len;
-    const float sigma = sched[3 * n], tau = sched[3 * n + 1], factor = sched[3 * n + 2];
+    const float sigma = sched[4 * n], tau = sched[4 * n + 1], factor = sched[4 * n + 2], hub = sched[4 * n + 3];
     float st_du = 0.0f, st_dp = 0.0f, st_rho = 0.0f, st_tv = 0.0f;
 #pragma unroll
     for (int j = 0; j < V; ++j)
@@ -765,10 +769,10 @@ __global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __rest
             const float inc = sigma * dq[e];
             pv[e] = p[j][3 * c + e] + inc;
           }
-          prox_dual_row(pv, wtv);
+          prox_dual_row(pv, wtv, hub);
           if (wtv_zero) { pv[0] = 0.0f; pv[1] = 0.0f; pv[2] = 0.0f; }
           if (part) {
-            st_tv += __builtin_amdgcn_sqrtf(dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2]);
+            st_tv += huber_of_norm(__builtin_amdgcn_sqrtf(dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2]), eps);
 #pragma unroll
             for (int e = 0; e < 3; ++e) {
               const float dd = pv[e] - p[j][3 * c + e];
@@ -918,6 +922,7 @@ struct svs_plan {
   std::vector<int> dims;                  // 3 per instance (NZ, NY, NX)
   std::vector<svs::LevelPlan> levels;
   std::vector<float> sig, tau, factor;    // per iteration, rounded to float
+  std::vector<float> hub;                 // per iteration: 1 + sigma eps / weight_tv (1: the reference's TV)
   std::vector<float> blur_w;              // K^3 weights (host copy)
   float* blur_dev = nullptr;
   char* ws = nullptr;
@@ -938,7 +943,7 @@ struct svs_plan {
   size_t table_bytes = 0;                // all levels' LevelDesc tables
   hipEvent_t staged[kStagingSlots] = {};
   int next_slot = 0;
-  float* sched_dev = nullptr;            // (sigma, tau, tau * lambda) per iteration
+  float* sched_dev = nullptr;            // (sigma, tau, tau * lambda, Huber factor) per iteration
   // Per-iteration statistics (svs_tvl1_plan_set_stats): the caller's output
   // and the plan's own buffer of the sweeps' partial sums for one warp.
   float* stats_out = nullptr;
@@ -1225,7 +1230,7 @@ static int plan_layout(svs_plan* pl, char* base, size_t* total_out) {
       }
   }
   if (P.use_blur) pl->blur_dev = (float*)take(sizeof(float) * pl->blur_w.size());
-  pl->sched_dev = (float*)take(sizeof(float) * 3 * (size_t)std::max(1, P.max_outer_iterations));
+  pl->sched_dev = (float*)take(sizeof(float) * 4 * (size_t)std::max(1, P.max_outer_iterations));
   *total_out = align_up(off, 256);
   return 0;
 }
@@ -1237,6 +1242,7 @@ static int validate_params(const svs_tvl1_params* P) {
   SVS_REQUIRE(valid_bc(P->boundary) && valid_ip(P->interpolation), "invalid interpolation/boundary type");
   SVS_REQUIRE(!P->use_blur || (P->blur_K >= 1 && (P->blur_K & 1) && P->blur_weights),
               "blur enabled but kernel size/weights invalid");
+  SVS_REQUIRE(P->huber_eps >= 0.0 && std::isfinite(P->huber_eps), "huber_eps must be finite and >= 0");
   return 0;
 }
 
@@ -1271,13 +1277,17 @@ static int validate_dims(const svs_tvl1_params* P, int ninst, const int* dims) {
 static void fill_schedule(svs_plan* pl) {
   const svs_tvl1_params& P = pl->P;
   const int K = P.max_outer_iterations;
-  pl->sig.resize(K); pl->tau.resize(K); pl->factor.resize(K);
+  pl->sig.resize(K); pl->tau.resize(K); pl->factor.resize(K); pl->hub.resize(K);
   // tvl13d.py:208-246: Python floats, reset per warp, rounded to float at use.
   double sg = P.sigma, ta = P.tau;
   for (int n = 0; n < K; ++n) {
     pl->sig[n] = (float)sg;
     pl->tau[n] = (float)ta;
     pl->factor[n] = (float)ta * P.weight_matching;  // opticalFlowOps.cu:442, FP32 product
+    // Huber-TV (eps > 0; no reference counterpart): 1 + sigma_n eps / weight_tv
+    // in double from the Python-float sigma_n, used as a float.
+    pl->hub[n] = (P.huber_eps == 0.0 || P.weight_tv == 0.0f) ? 1.0f
+                                                              : (float)(1.0 + sg * P.huber_eps / (double)P.weight_tv);
     if (P.pd_type == 2) {
       const double th = 1. / std::sqrt(1. + 2. * P.gamma * ta);
       ta *= th;
@@ -1397,11 +1407,12 @@ static int upload_static(svs_plan* pl, hipStream_t st) {
     SVS_HIP(hipStreamSynchronize(st));
   }
   if (pl->P.max_outer_iterations > 0) {
-    std::vector<float> sched(3 * (size_t)pl->P.max_outer_iterations);
+    std::vector<float> sched(4 * (size_t)pl->P.max_outer_iterations);
     for (int n = 0; n < pl->P.max_outer_iterations; ++n) {
-      sched[3 * n] = pl->sig[n];
-      sched[3 * n + 1] = pl->tau[n];
-      sched[3 * n + 2] = pl->factor[n];
+      sched[4 * n] = pl->sig[n];
+      sched[4 * n + 1] = pl->tau[n];
+      sched[4 * n + 2] = pl->factor[n];
+      sched[4 * n + 3] = pl->hub[n];
     }
     SVS_HIP(hipMemcpyAsync(pl->sched_dev, sched.data(), sizeof(float) * sched.size(), hipMemcpyHostToDevice, st));
     SVS_HIP(hipStreamSynchronize(st));
@@ -1462,22 +1473,31 @@ static int upload_tiles(svs_plan* pl, int s, hipStream_t st) {
   return 0;
 }
 
-template <int BC, bool ANISO, bool STATS>
-static void launch_sweep_s(const LevelPlan& L, int parity, float sg, float ta, float fac, float wtv, int wz,
-                           float4* part, hipStream_t st) {
+// The scalars of one sweep: schedule position n's steps and Huber factor.
+struct SweepScalars {
+  float sg, ta, fac, wtv, hub, eps;
+};
+
+template <int BC, bool ANISO, bool STATS, bool HUBER>
+static void launch_sweep_s(const LevelPlan& L, int parity, const SweepScalars& a, float4* part, hipStream_t st) {
   if (L.unit_h)
-    hipLaunchKernelGGL((k_sweep<BC, ANISO, true, STATS>), dim3(L.ntiles), dim3(L.threads), L.lds, st, L.dev,
-                       L.tiles, parity, sg, ta, fac, wtv, wz, part);
+    hipLaunchKernelGGL((k_sweep<BC, ANISO, true, STATS, HUBER>), dim3(L.ntiles), dim3(L.threads), L.lds, st, L.dev,
+                       L.tiles, parity, a.sg, a.ta, a.fac, a.wtv, a.hub, a.eps, part);
   else
-    hipLaunchKernelGGL((k_sweep<BC, ANISO, false, STATS>), dim3(L.ntiles), dim3(L.threads), L.lds, st, L.dev,
-                       L.tiles, parity, sg, ta, fac, wtv, wz, part);
+    hipLaunchKernelGGL((k_sweep<BC, ANISO, false, STATS, HUBER>), dim3(L.ntiles), dim3(L.threads), L.lds, st,
+                       L.dev, L.tiles, parity, a.sg, a.ta, a.fac, a.wtv, a.hub, a.eps, part);
 }
 
 template <int BC, bool ANISO>
-static void launch_sweep_t(const LevelPlan& L, int parity, float sg, float ta, float fac, float wtv, int wz,
-                           float4* part, hipStream_t st) {
-  if (part) launch_sweep_s<BC, ANISO, true>(L, parity, sg, ta, fac, wtv, wz, part, st);
-  else launch_sweep_s<BC, ANISO, false>(L, parity, sg, ta, fac, wtv, wz, nullptr, st);
+static void launch_sweep_t(const LevelPlan& L, int parity, const SweepScalars& a, float4* part, hipStream_t st) {
+  const bool hub = a.eps > 0.0f;
+  if (part) {
+    if (hub) launch_sweep_s<BC, ANISO, true, true>(L, parity, a, part, st);
+    else launch_sweep_s<BC, ANISO, true, false>(L, parity, a, part, st);
+  } else {
+    if (hub) launch_sweep_s<BC, ANISO, false, true>(L, parity, a, nullptr, st);
+    else launch_sweep_s<BC, ANISO, false, false>(L, parity, a, nullptr, st);
+  }
 }
 
 // One sweep at schedule position n; part (null: no statistics) receives one
@@ -1485,17 +1505,15 @@ static void launch_sweep_t(const LevelPlan& L, int parity, float sg, float ta, f
 static void launch_sweep(const svs_plan* pl, const LevelPlan& L, int parity, int n, hipStream_t st,
                          float4* part = nullptr) {
   const svs_tvl1_params& P = pl->P;
-  const float sg = pl->sig[n], ta = pl->tau[n], fac = pl->factor[n];
-  const float wtv = P.weight_tv;
-  const int wz = P.weight_tv == 0.0f;
+  const SweepScalars a{pl->sig[n], pl->tau[n], pl->factor[n], P.weight_tv, pl->hub[n], (float)P.huber_eps};
   const bool an = P.use_aniso != 0;
   switch (P.boundary * 2 + (an ? 1 : 0)) {
-    case 0: launch_sweep_t<BD_NEAREST, false>(L, parity, sg, ta, fac, wtv, wz, part, st); break;
-    case 1: launch_sweep_t<BD_NEAREST, true>(L, parity, sg, ta, fac, wtv, wz, part, st); break;
-    case 2: launch_sweep_t<BD_MIRROR, false>(L, parity, sg, ta, fac, wtv, wz, part, st); break;
-    case 3: launch_sweep_t<BD_MIRROR, true>(L, parity, sg, ta, fac, wtv, wz, part, st); break;
-    case 4: launch_sweep_t<BD_REFLECT, false>(L, parity, sg, ta, fac, wtv, wz, part, st); break;
-    default: launch_sweep_t<BD_REFLECT, true>(L, parity, sg, ta, fac, wtv, wz, part, st); break;
+    case 0: launch_sweep_t<BD_NEAREST, false>(L, parity, a, part, st); break;
+    case 1: launch_sweep_t<BD_NEAREST, true>(L, parity, a, part, st); break;
+    case 2: launch_sweep_t<BD_MIRROR, false>(L, parity, a, part, st); break;
+    case 3: launch_sweep_t<BD_MIRROR, true>(L, parity, a, part, st); break;
+    case 4: launch_sweep_t<BD_REFLECT, false>(L, parity, a, part, st); break;
+    default: launch_sweep_t<BD_REFLECT, true>(L, parity, a, part, st); break;
   }
 }
 
@@ -1506,12 +1524,13 @@ static void launch_resident_v(const svs_plan* pl, const LevelPlan& L, int parity
   const float wtv = P.weight_tv;
   const int wz = P.weight_tv == 0.0f;
   const int len = P.max_outer_iterations;
+  const float eps = (float)P.huber_eps;
   if (L.unit_h)
     hipLaunchKernelGGL((k_sweep_resident<BC, ANISO, true, V>), dim3(pl->ninst), dim3(L.res_threads), L.res_lds, st,
-                       L.dev, pl->sched_dev, first, K, len, parity, wtv, wz, part);
+                       L.dev, pl->sched_dev, first, K, len, parity, wtv, wz, eps, part);
   else
     hipLaunchKernelGGL((k_sweep_resident<BC, ANISO, false, V>), dim3(pl->ninst), dim3(L.res_threads), L.res_lds,
-                       st, L.dev, pl->sched_dev, first, K, len, parity, wtv, wz, part);
+                       st, L.dev, pl->sched_dev, first, K, len, parity, wtv, wz, eps, part);
 }
 
 template <int BC, bool ANISO>
@@ -1547,15 +1566,21 @@ static int allow_resident_lds() {
   return 0;
 }
 
-template <bool STATS>
+template <bool STATS, bool HUBER>
 static int allow_sweep_lds(int v) {
   const void* kernels[] = {
-      (const void*)k_sweep<BD_NEAREST, false, false, STATS>, (const void*)k_sweep<BD_NEAREST, true, false, STATS>,
-      (const void*)k_sweep<BD_MIRROR, false, false, STATS>,  (const void*)k_sweep<BD_MIRROR, true, false, STATS>,
-      (const void*)k_sweep<BD_REFLECT, false, false, STATS>, (const void*)k_sweep<BD_REFLECT, true, false, STATS>,
-      (const void*)k_sweep<BD_NEAREST, false, true, STATS>,  (const void*)k_sweep<BD_NEAREST, true, true, STATS>,
-      (const void*)k_sweep<BD_MIRROR, false, true, STATS>,   (const void*)k_sweep<BD_MIRROR, true, true, STATS>,
-      (const void*)k_sweep<BD_REFLECT, false, true, STATS>,  (const void*)k_sweep<BD_REFLECT, true, true, STATS>};
+      (const void*)k_sweep<BD_NEAREST, false, false, STATS, HUBER>,
+      (const void*)k_sweep<BD_NEAREST, true, false, STATS, HUBER>,
+      (const void*)k_sweep<BD_MIRROR, false, false, STATS, HUBER>,
+      (const void*)k_sweep<BD_MIRROR, true, false, STATS, HUBER>,
+      (const void*)k_sweep<BD_REFLECT, false, false, STATS, HUBER>,
+      (const void*)k_sweep<BD_REFLECT, true, false, STATS, HUBER>,
+      (const void*)k_sweep<BD_NEAREST, false, true, STATS, HUBER>,
+      (const void*)k_sweep<BD_NEAREST, true, true, STATS, HUBER>,
+      (const void*)k_sweep<BD_MIRROR, false, true, STATS, HUBER>,
+      (const void*)k_sweep<BD_MIRROR, true, true, STATS, HUBER>,
+      (const void*)k_sweep<BD_REFLECT, false, true, STATS, HUBER>,
+      (const void*)k_sweep<BD_REFLECT, true, true, STATS, HUBER>};
   for (const void* k : kernels) SVS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, v));
   return 0;
 }
@@ -1565,8 +1590,10 @@ static int allow_large_lds(const svs_plan* pl) {
   // shape_ok keeps every tile within 96 KiB; enable that much once.
   (void)pl;
   const int v = 96 * 1024;
-  if (int rc = allow_sweep_lds<false>(v)) return rc;
-  if (int rc = allow_sweep_lds<true>(v)) return rc;
+  if (int rc = allow_sweep_lds<false, false>(v)) return rc;
+  if (int rc = allow_sweep_lds<true, false>(v)) return rc;
+  if (int rc = allow_sweep_lds<false, true>(v)) return rc;
+  if (int rc = allow_sweep_lds<true, true>(v)) return rc;
   // The resident sweep holds 48 bytes per voxel of up to 2048 voxels.
   if (int rc = allow_resident_lds<BD_NEAREST, false>()) return rc;
   if (int rc = allow_resident_lds<BD_NEAREST, true>()) return rc;

@@ -314,14 +314,17 @@ __device__ __forceinline__ void tangent1(const float n[3], float t1[3]) {
   }
 }
 
-// opticalFlowOps.cu:187-205 — projection of one row onto the l2 ball.
-__device__ __forceinline__ void prox_dual_row(float v[3], float wtv) {
+// opticalFlowOps.cu:187-205 — projection of one row onto the l2 ball.  hub
+// (not in the reference, which is hub = 1): the Huber-TV step, v / max(hub,
+// ||v|| / wtv) with hub = 1 + sigma eps / wtv, i.e. v / hub projected onto the
+// ball, in one division; hub = 1 gives the reference's bits.
+__device__ __forceinline__ void prox_dual_row(float v[3], float wtv, float hub = 1.0f) {
   float ns = 0.0f;
 #pragma unroll
   for (int d = 0; d < 3; ++d) ns += v[d] * v[d];
-  const float den = fmaxf(1.0f, sqrtf(ns) / wtv);
-  // Inside the ball den == 1 and v / 1 == v exactly: a wave whose rows all lie
-  // inside skips the three divisions.
+  const float den = fmaxf(hub, sqrtf(ns) / wtv);
+  // Inside the ball (hub = 1) den == 1 and v / 1 == v exactly: a wave whose
+  // rows all lie inside skips the three divisions.
   if (den > 1.0f) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) v[d] = v[d] / den;
@@ -383,6 +386,13 @@ __device__ __forceinline__ svs_f4 bld4(svs_rsrc r, uint32_t voff, uint32_t soff)
 }
 __device__ __forceinline__ void bst(svs_rsrc r, uint32_t voff, uint32_t soff, float v) {
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, wave_uniform(soff), 0);
+}
+
+// The Huber function of a row norm t for the energy statistics: t^2 / (2 eps)
+// up to eps, t - eps / 2 beyond; the identity for eps = 0.
+__device__ __forceinline__ float huber_of_norm(float t, float eps) {
+  if (eps == 0.0f) return t;
+  return t <= eps ? t * t / (2.0f * eps) : t - 0.5f * eps;
 }
 
 // Sum over the 64 lanes of a wave, returned to every lane.  The butterfly

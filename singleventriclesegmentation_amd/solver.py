@@ -41,6 +41,9 @@ class SolverConfig:
     length_fixed: bool = False
     length: Tuple[float, float, float] = (1.0, 1.0, 1.0)
     sweep_shape: Tuple[int, int, int] = (0, 0, 0)
+    # Huber-TV smoothing (not in the reference, whose TV is eps = 0): each dual
+    # row becomes v_c / max(1 + sigma_n eps / weight_tv, ||v_c|| / weight_tv).
+    huber_eps: float = 0.0
 
     def native(self, use_initial_state=False):
         p = TVL1Params()
@@ -66,6 +69,7 @@ class SolverConfig:
         p.LZ, p.LY, p.LX = float(np.float32(lz)), float(np.float32(ly)), float(np.float32(lx))
         p.use_initial_state = 1 if use_initial_state else 0
         p.sweep_bx, p.sweep_by, p.sweep_cz = (int(v) for v in self.sweep_shape)
+        p.huber_eps = float(self.huber_eps)
         return p, weights  # keep `weights` alive while `p` is used
 
     def level_shapes(self, shape):

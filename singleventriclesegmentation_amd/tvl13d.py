@@ -48,6 +48,7 @@ class TVL13DOpticalFlow:
         use_graph: bool = True,
         autotune: bool = False,
         record_stats: bool = False,
+        huber_eps: float = 0.0,
     ):
         """Arguments as in tvl13d.py:14-52; ``length`` is (lx, ly, lz).
 
@@ -62,7 +63,11 @@ class TVL13DOpticalFlow:
         ||u^{n+1} - u^n|| (tvl13d.py:237; also its ||z - z_old||, :253, since
         z = u) and "dual" = ||p^{n+1} - p^n|| (:222), and "energy" = lambda
         sum|rho(u^n)| + lambda_tv sum_c ||D grad u^n_c|| of the iterate
-        entering each iteration."""
+        entering each iteration; ``huber_eps`` > 0 turns the TV term into a
+        Huber-TV one (the reference's TV is eps = 0): each dual row becomes
+        v_c / max(1 + sigma_n eps / weight_tv, ||v_c|| / weight_tv), and the
+        energy uses the Huber function of the row norms.  compute_unfused
+        follows the reference's operators and supports eps = 0 only."""
         self.num_scales = num_scales
         self.max_warps = max_warps
         self.max_outer_iterations = max_outer_iterations
@@ -84,6 +89,9 @@ class TVL13DOpticalFlow:
         self.autotune = autotune
         self.record_stats = record_stats
         self.stats = None
+        if not huber_eps >= 0.0:
+            raise ValueError("huber_eps must be >= 0")
+        self.huber_eps = float(huber_eps)
         self.config = SolverConfig(
             num_scales=num_scales, max_warps=max_warps, max_outer_iterations=max_outer_iterations,
             weight_matching=weight_matching, weight_tv=weight_tv, pd_type=primaldual_algorithm_type,
@@ -91,7 +99,7 @@ class TVL13DOpticalFlow:
             aniso_params=tuple(float(v) for v in anisotropic_diff_params), use_blur=bool(use_gaussian_blur),
             blur_sigma=float(gaussian_blur_sigma), interpolation=int(self.interpolation_type),
             boundary=int(self.boundary_type), length_fixed=(length_type == "fixed"),
-            length=(float(self.length_x), float(self.length_y), float(self.length_z)))
+            length=(float(self.length_x), float(self.length_y), float(self.length_z)), huber_eps=self.huber_eps)
         self._plans = {}
 
     # ------------------------------------------------------------ fused path
@@ -163,6 +171,8 @@ class TVL13DOpticalFlow:
 
     def compute_unfused(self, I0, I1, u, p):
         """tvl13d.py:122-162 step by step (one operator call per reference call)."""
+        if self.huber_eps != 0.0:
+            raise NotImplementedError("compute_unfused runs the reference's operators, whose TV has no Huber eps")
         I0s, I1s, us, ps, meshes = self.generate_pyramid(I0, I1, u, p)
         rec = None
         if self.record_stats:

@@ -25,10 +25,11 @@ def oracle_stats(O, I0, I1, S, W, K, boundary="MIRROR", aniso=True):
     return O.solve(O.blur(I0, w), O.blur(I1, w), prm, stats=True)
 
 
-def plan_stats(dev, frames, S, W, K, boundary="MIRROR", aniso=True, resident=None, graph=True, stats=True):
+def plan_stats(dev, frames, S, W, K, boundary="MIRROR", aniso=True, resident=None, graph=True, stats=True,
+               huber_eps=0.0):
     cfg = SolverConfig(num_scales=S, max_warps=W, max_outer_iterations=K, boundary={"NEAREST": 0, "MIRROR": 1,
                                                                                     "REFLECT": 2}[boundary],
-                       use_aniso=aniso)
+                       use_aniso=aniso, huber_eps=huber_eps)
     shapes = [f[0].shape for f in frames]
     plan = Plan(cfg, shapes, device=dev)
     outs = []
@@ -120,3 +121,23 @@ def test_stats_match_the_reference_driver(dev):
     for k in ("primal", "dual", "update", "energy"):
         np.testing.assert_allclose(fused[k], ref[k], rtol=1e-4)
     assert np.array_equal(fused["update"], fused["primal"])
+
+
+@pytest.mark.parametrize("boundary,aniso,eps", [("MIRROR", True, 0.05), ("NEAREST", False, 0.2),
+                                                ("REFLECT", True, 0.01)])
+def test_huber_tv_matches_oracle(O, dev, boundary, aniso, eps):
+    """Huber-TV (eps > 0, not in the reference): the fused solve (a marching and
+    a resident level) equals the oracle's bits, its statistics the oracle's
+    Huber energies and residuals, and TVL13DOpticalFlow(huber_eps=...) the plan."""
+    I0, I1, _ = pair((20, 24, 28), seed=9)
+    S, W, K = 2, 3, 8
+    w = gaussian_weights3d(1.0)
+    prm = O.solve_params(num_scales=S, max_warps=W, max_outer_iterations=K, boundary=boundary, use_aniso=aniso,
+                         huber_eps=eps)
+    uo, po, so = O.solve(O.blur(I0, w), O.blur(I1, w), prm, stats=True)
+    st, [(u, p)] = plan_stats(dev, [(I0, I1)], S, W, K, boundary, aniso, huber_eps=eps)
+    assert np.array_equal(u, uo) and np.array_equal(p, po)
+    np.testing.assert_allclose(st[:, :, :, 0], so, rtol=RTOL)
+    solver = make_solver(S, W, K, boundary_type=boundary, use_anisotropic_diff=aniso, huber_eps=eps)
+    uc, pc = solver.compute(torch.from_numpy(I0).to(dev), torch.from_numpy(I1).to(dev), None, None)
+    assert np.array_equal(uc.cpu().numpy(), uo) and np.array_equal(pc.cpu().numpy(), po)

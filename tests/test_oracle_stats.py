@@ -102,3 +102,43 @@ def test_identical_frames_have_zero_statistics(O):
     f = phantom.add_noise(phantom.clean_frame((10, 10, 10)), 3)
     _, _, st = O.solve(f, f, O.solve_params(num_scales=2, max_warps=2, max_outer_iterations=3), stats=True)
     assert not st.any()
+
+
+def test_huber_dual_step_and_energy_of_a_ramp(O):
+    """Huber-TV (eps > 0, not in the reference): from p^0 = 0 the first dual row
+    is v_c / max(hub, |v_c| / lambda_tv) with v_c = (sigma_0 alpha_c, 0, 0) and
+    hub = 1 + sigma_0 eps / lambda_tv -- scaled by 1/hub inside the ball,
+    projected outside -- and the energy of the ramp is lambda_tv |interior|
+    sum_c H_eps(|alpha_c|)."""
+    shape = (5, 6, 8)
+    I = np.full(shape, 0.25, np.float32)
+    alpha = np.array([0.25, -0.5, 0.125])
+    u0 = np.zeros(shape + (3,), np.float32)
+    u0[...] = (alpha[None, None, None, :] * np.arange(shape[2], dtype=np.float32)[None, None, :, None])
+    interior = shape[0] * shape[1] * (shape[2] - 2)
+    H = lambda t, e: t * t / (2 * e) if t <= e else t - e / 2
+    for wtv, sigma, eps in ((0.1, 0.5, 0.05), (1.0, 0.5, 0.5), (0.025, 0.5, 0.3)):
+        prm = O.solve_params(num_scales=1, max_warps=1, max_outer_iterations=1, weight_tv=wtv, use_aniso=False,
+                             pd_params=(sigma, 0.5, 0.25, 1.0), huber_eps=eps)
+        _, p1, st = O.solve(I, I, prm, u=u0, p=np.zeros(shape + (3, 3), np.float32), stats=True)
+        w = float(np.float32(wtv))
+        hub = 1.0 + sigma * eps / w
+        rows = sigma * np.abs(alpha) / np.maximum(hub, sigma * np.abs(alpha) / w)
+        np.testing.assert_allclose(st[0, 0, 0, 1], np.sqrt(interior * np.sum(rows ** 2)), rtol=1e-6)
+        # an interior voxel's rows: (sigma alpha_c / den_c, 0, 0)
+        got = p1[2, 3, 4].astype(np.float64)
+        np.testing.assert_allclose(got[:, 0], sigma * alpha / np.maximum(hub, sigma * np.abs(alpha) / w), rtol=1e-6)
+        assert not got[:, 1:].any()
+        want_e = w * interior * sum(H(abs(a), eps) for a in alpha)
+        np.testing.assert_allclose(st[0, 0, 0, 2], want_e, rtol=1e-12)
+
+
+def test_huber_eps_zero_is_the_reference_step(O):
+    """eps = 0 and a 19-entry parameter pack (before eps existed) give the same bits."""
+    I0, I1 = pair()
+    prm = O.solve_params(num_scales=2, max_warps=2, max_outer_iterations=4)
+    a = O.solve(I0, I1, prm)
+    b = O.solve(I0, I1, prm[:19])
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    c = O.solve(I0, I1, O.solve_params(num_scales=2, max_warps=2, max_outer_iterations=4, huber_eps=0.05))
+    assert not np.array_equal(a[0], c[0])
