@@ -141,3 +141,34 @@ def test_huber_tv_matches_oracle(O, dev, boundary, aniso, eps):
     solver = make_solver(S, W, K, boundary_type=boundary, use_anisotropic_diff=aniso, huber_eps=eps)
     uc, pc = solver.compute(torch.from_numpy(I0).to(dev), torch.from_numpy(I1).to(dev), None, None)
     assert np.array_equal(uc.cpu().numpy(), uo) and np.array_equal(pc.cpu().numpy(), po)
+
+
+def test_stats_follow_shape_changes_and_toggling(O, dev):
+    """Statistics stay right when the launch shapes change after they were
+    turned on (more tiles: the plan grows its partial-sum buffer and captures
+    its graph again) and when they are turned off and on again."""
+    I0, I1, _ = pair((24, 24, 24), seed=13)
+    S, W, K = 2, 2, 6
+    _, _, so = oracle_stats(O, I0, I1, S, W, K)
+    cfg = SolverConfig(num_scales=S, max_warps=W, max_outer_iterations=K)
+    plan = Plan(cfg, [I0.shape], device=dev)
+    a, b = torch.from_numpy(I0).to(dev), torch.from_numpy(I1).to(dev)
+    u = torch.empty(I0.shape + (3,), device=dev)
+    plan.bind(0, a, b, u_out=u)
+    plan.set_stats(True)
+    plan.run()
+    first = plan.stats.clone()
+    tiles0 = plan.info(0)["tiles"]
+    plan.set_shapes([(8, 4, 1), (8, 4, 1)])  # 6 x 2 interiors, one plane: more tiles
+    assert plan.info(0)["tiles"] > 2 * tiles0
+    plan.run()
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(plan.stats.cpu().numpy()[:, :, :, 0], so, rtol=RTOL)
+    np.testing.assert_allclose(plan.stats.cpu().numpy(), first.cpu().numpy(), rtol=RTOL)
+    plan.set_stats(False)
+    plan.run()
+    plan.set_stats(True)
+    plan.run(use_graph=False)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(plan.stats.cpu().numpy()[:, :, :, 0], so, rtol=RTOL)
+    plan.close()
