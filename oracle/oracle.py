@@ -15,7 +15,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libsvs_oracle.so")
+# SVS_ORACLE_LIB selects another build of the same source (the sanitizer
+# build of `make sanitize`, for tests/test_oracle_sanitized.py).
+_LIB_PATH = os.environ.get("SVS_ORACLE_LIB") or os.path.join(_HERE, "libsvs_oracle.so")
 _lib = None
 
 BOUNDARY = {"NEAREST": 0, "MIRROR": 1, "REFLECT": 2}

@@ -11,7 +11,9 @@ import os
 import torch  # noqa: F401  (must precede loading the HIP library)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsvsflow.so")
+# SVS_FLOW_LIB selects another build of the same sources (the host-sanitized
+# one of build.py --host-sanitize); either way there is no fallback.
+LIB_PATH = os.environ.get("SVS_FLOW_LIB") or os.path.join(_HERE, "libsvsflow.so")
 
 c_float_p = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p

@@ -31,7 +31,7 @@ from .warping import Warp, propagate
 Shape = Tuple[int, int, int]
 
 
-@dataclass
+@dataclass(eq=False)   # requests compare by identity (they hold tensors)
 class Request:
     rid: int
     kind: str                      # "pair" | "sequence"
@@ -45,7 +45,8 @@ class Request:
     flows: Optional[torch.Tensor] = None          # pair [Z,Y,X,3]; sequence [2(T-1),Z,Y,X,3]
     propagated: Optional[Tuple[torch.Tensor, torch.Tensor]] = None
     admitted: Optional[float] = None   # server clock when its unit was admitted
-    done: Optional[float] = None
+    done: Optional[float] = None       # server clock when it completed or was withdrawn
+    cancelled: bool = False
 
     @property
     def solves(self):
@@ -102,6 +103,7 @@ class FlowServer:
         self.used = 0            # bytes held by resident plans and in-flight outputs
         self.warper = Warp("CUBIC_HERMITESPLINE", "MIRROR", "numDofs", (1, 1, 1))
         self.completed: List[Request] = []
+        self.cancelled: List[Request] = []
 
     # ------------------------------------------------------------ memory
     def _ws(self, key):
@@ -184,6 +186,28 @@ class FlowServer:
         req.ready = torch.cuda.Event()
         req.ready.record(torch.cuda.current_stream(self.device))
         self.pending.append(req)
+
+    def cancel(self, req: Request, now: float = 0.0) -> bool:
+        """Withdraws a request.  A pending one leaves the queue and never runs.
+        An admitted one keeps running with the rest of its unit -- the device
+        work of a batched plan cannot be cut short without touching the
+        requests batched with it -- and its results are dropped when the unit
+        retires, which also returns its memory as usual.  Either way the other
+        requests' results are unchanged and the request is not completed.
+        Returns False if it already completed, was already withdrawn or is not
+        known to this server."""
+        if req.cancelled or req.done is not None:
+            return False
+        if any(r is req for r in self.pending):
+            self.pending.remove(req)
+            req.cancelled = True
+            req.done = now
+            self.cancelled.append(req)
+            return True
+        if any(r is req for u in self.running for r in u.requests):
+            req.cancelled = True   # dropped in _retire
+            return True
+        return False
 
     def _next_unit(self) -> Optional[Tuple[List[Request], Shape, int]]:
         if not self.pending:
@@ -272,11 +296,17 @@ class FlowServer:
                 continue
             k = 0
             for r in u.requests:
+                if r.cancelled:   # withdrawn while its unit ran: results dropped
+                    k += r.solves
+                    r.propagated = None
+                    r.done = now
+                    self.cancelled.append(r)
+                    continue
                 if r.kind == "pair":
                     r.flows = u.outputs[k]
                     k += 1
                 else:
-                    r.flows = u.outputs[:r.solves]
+                    r.flows = u.outputs[k:k + r.solves]
                     k += r.solves
                     if r.propagated is not None:
                         # Made on the slot stream: their blocks must not go back

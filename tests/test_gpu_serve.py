@@ -113,3 +113,33 @@ def test_sequence_without_masks_from_fresh_frames(dev):
         assert torch.equal(solver.compute(frames[t + 1], frames[t], None, None)[0], req.flows[t])
         assert torch.equal(solver.compute(frames[t], frames[t + 1], None, None)[0], req.flows[FRAMES - 1 + t])
     server.close()
+
+
+def test_cancelled_requests_leave_the_others_unchanged(dev):
+    """Requests withdrawn mid-flight (one pair of a batched unit, one sequence
+    unit) or while pending never complete and return their memory; every other
+    request still equals the plain drop-in solving it alone."""
+    cfg = SolverConfig(num_scales=2, max_warps=2, max_outer_iterations=4)
+    pool = make_pool(SHAPES, FRAMES, dev)
+    reqs = poisson_requests(0, 10, SHAPES, pool, FRAMES, seed=11, p_pair=0.7, mask_last=3)
+    server = FlowServer(cfg, device=dev, slots=2, pair_buckets=(1, 2, 4))
+    for r in reqs:
+        server.submit(r)
+    server.step(0.0)                       # fills both slots; the rest stay pending
+    assert len(server.running) == 2 and server.pending
+    running = [r for u in server.running for r in u.requests]
+    mid = [running[0]] + ([running[-1]] if running[-1] is not running[0] else [])
+    pend = list(server.pending)[:2]
+    for r in mid + pend:
+        assert server.cancel(r, now=0.0)
+    while not server.idle():
+        server.step(0.0)
+    torch.cuda.synchronize(dev)
+    withdrawn = {id(r) for r in mid + pend}
+    assert {id(r) for r in server.cancelled} == withdrawn
+    assert all(r.flows is None for r in mid + pend)
+    done = [r for r in reqs if id(r) not in withdrawn]
+    assert {id(r) for r in server.completed} == {id(r) for r in done}
+    assert server.used == sum(p.workspace_bytes for _, p in server.idle_plans)
+    check_against_alone(done, dev)
+    server.close()

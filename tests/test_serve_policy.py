@@ -48,3 +48,29 @@ def test_sequence_cap():
     s = server([seq(0)], running_kinds=("sequence", "pair"), sequence_slots=2)
     assert [r.rid for r in s._next_unit()[0]] == [0]
     assert server([]) ._next_unit() is None
+
+
+def test_cancel_pending_and_running():
+    """A pending request leaves the queue; an admitted one is dropped when its
+    unit retires while the request batched with it completes; completed,
+    withdrawn and unknown requests cannot be cancelled."""
+    import torch
+    from collections import deque
+    r1, r2, r3, r4 = pair(1), pair(2), seq(3), pair(4)
+    s = server([r2, r3])
+    s.cancelled, s.completed = [], []
+    assert s.cancel(r2, now=1.5) and r2.cancelled and r2.done == 1.5
+    assert [r.rid for r in s.pending] == [3] and s.cancelled == [r2]
+    assert not s.cancel(r2)                      # already withdrawn
+    assert not s.cancel(pair(9))                 # unknown
+    out = torch.arange(2 * 2 * 2 * 2 * 3, dtype=torch.float32).reshape(2, 2, 2, 2, 3)
+    unit = SimpleNamespace(requests=[r1, r4], event=SimpleNamespace(query=lambda: True), outputs=out, nbytes=7,
+                           key=("k",), plan="plan", slot=0)
+    s.running = [unit]
+    s.used, s.idle_plans, s.free_slots = 7, [], deque()
+    assert s.cancel(r1) and r1.done is None      # running: still on the device
+    s._retire(now=2.0)
+    assert s.completed == [r4] and torch.equal(r4.flows, out[1])
+    assert r1.flows is None and r1.done == 2.0 and s.cancelled == [r2, r1]
+    assert s.used == 0 and list(s.free_slots) == [0] and s.idle_plans == [(("k",), "plan")]
+    assert not s.cancel(r4)                      # completed
