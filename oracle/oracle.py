@@ -334,16 +334,19 @@ def spatial_augment(x, R=None, flip_axes=(), axes=None, mode="bilinear", padding
 def solve_params(num_scales=4, max_warps=10, max_outer_iterations=50, weight_matching=1.0, weight_tv=0.025,
                  pd_type=2, pd_params=(0.5, 0.5, 0.25, 1.0), use_aniso=True, aniso_params=(5.0, 0.5),
                  interpolation="CUBIC_HERMITESPLINE", boundary="MIRROR", length_type="numDofs",
-                 length=(1, 1, 1), huber_eps=0.0):
+                 length=(1, 1, 1), huber_eps=0.0, relax_theta=0.0):
     """Packs the TVL13DOpticalFlow constructor arguments (tvl13d.py:14-67).
 
     ``length`` is (lx, ly, lz) as in conf/flow.yaml; the packed order is LZ, LY, LX.
     ``huber_eps`` (not in the reference, whose TV is eps = 0) selects the
-    Huber-TV dual step (svs_oracle.cpp, prox_dual)."""
+    Huber-TV dual step (svs_oracle.cpp, prox_dual); ``relax_theta`` (not in the
+    reference, whose z = u) the over-relaxation of the dual step's point
+    (svs_oracle.cpp, solve_level)."""
     lx, ly, lz = length
     return np.array([num_scales, max_warps, max_outer_iterations, weight_matching, weight_tv, pd_type,
                      *pd_params, 1.0 if use_aniso else 0.0, *aniso_params, INTERP[interpolation],
-                     BOUNDARY[boundary], 1.0 if length_type == "fixed" else 0.0, lz, ly, lx, huber_eps],
+                     BOUNDARY[boundary], 1.0 if length_type == "fixed" else 0.0, lz, ly, lx, huber_eps,
+                     relax_theta],
                     dtype=np.float64)
 
 
@@ -370,8 +373,8 @@ def solve(I0, I1, params, u=None, p=None, dtype=np.float32, stats=False):
     uo = np.empty_like(u)
     po = np.empty_like(p)
     params = np.ascontiguousarray(params, dtype=np.float64)
-    if params.size == 19:  # packed before huber_eps existed (golden fixtures): eps = 0
-        params = np.append(params, 0.0)
+    if params.size < 21:  # packed before huber_eps / relax_theta existed (golden fixtures): 0
+        params = np.append(params, np.zeros(21 - params.size))
     if not stats:
         _fn("solve", dtype)(_p(params), _p(I0), _p(I1), NZ, NY, NX, _p(u), _p(p), _p(uo), _p(po))
         return uo, po

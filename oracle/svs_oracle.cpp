@@ -656,6 +656,7 @@ struct SolveParams {
   int length_fixed;          // 0: numDofs (L = N-1), 1: fixed lengths below
   double LZ, LY, LX;
   double huber_eps;          // 0: the reference's TV (see prox_dual)
+  double relax_theta;        // 0: the reference's z = u (see solve_level)
 };
 
 // Per-iteration Huber factor hub_n = 1 + sigma_n eps / lambda_tv, formed in
@@ -700,18 +701,25 @@ template <class T> static Mesh<T> level_mesh(const SolveParams& P, int NZ, int N
 // the identity for the reference's eps = 0),
 //   E(u^n) = lambda * sum_i |rho(u^n)_i| + lambda_tv * sum_i sum_c H_eps(||(D grad u^n)_{i,c}||_2),
 // with rho and D grad u^n exactly as the iteration evaluates them, and lambda,
-// lambda_tv the FP32 weights.  Sums and norms are taken in double.
+// lambda_tv the FP32 weights.  Sums and norms are taken in double.  With
+// over-relaxation (theta != 0) the dual step differentiates z, not u^n, so the
+// energy is not available and is NaN.
 struct IterStats {
   double du2 = 0, dp2 = 0, rho_abs = 0, tv = 0;
 };
-static void store_stats(const IterStats& a, double lam, double wtv, double* out) {
+static void store_stats(const IterStats& a, double lam, double wtv, bool relaxed, double* out) {
   out[0] = std::sqrt(a.du2);
   out[1] = std::sqrt(a.dp2);
-  out[2] = lam * a.rho_abs + wtv * a.tv;
+  out[2] = relaxed ? std::nan("") : lam * a.rho_abs + wtv * a.tv;
 }
 
 // One pyramid level, tvl13d.py:164-255.  stats (optional) receives
 // [max_warps][max_iters][3] per-iteration statistics (see IterStats).
+// relax_theta (not in the reference, whose "relaxation" is z = u, tvl13d.py:
+// 251-252): the dual step differentiates z = u^n + theta (u^n - u^{n-1}),
+// formed after each primal step as d = u' - u, e = theta * d, z = u' + e in the
+// solve's precision with theta rounded to it, and reset to u at every warp's
+// first iteration (like sigma and tau).
 template <class T> static void solve_level(const SolveParams& P, const Mesh<T>& m, const T* I0, const T* I1,
                                            std::vector<T>& u, std::vector<T>& p, double* stats = nullptr) {
   const size_t N = (size_t)m.NZ * m.NY * m.NX;
@@ -723,6 +731,9 @@ template <class T> static void solve_level(const SolveParams& P, const Mesh<T>& 
   std::vector<T> I1w(N), Gw(3 * N), rho_c(N), rho(N), zg(9 * N), Dz(9 * N), dual(9 * N), pn(9 * N),
       Dp(9 * N), div(3 * N), primal(3 * N), un(3 * N);
   const T lam = (T)P.weight_matching, wtv = (T)P.weight_tv;
+  const bool relax = P.relax_theta != 0.0;
+  const T theta = (T)P.relax_theta;
+  std::vector<T> z(relax ? 3 * N : 0);
   for (int w = 0; w < P.max_warps; ++w) {
     warp(I1, 1, u.data(), m.NZ, m.NY, m.NX, m.hZ, m.hY, m.hX, P.ip, P.bc, I1w.data());
     warp(G.data(), 3, u.data(), m.NZ, m.NY, m.NX, m.hZ, m.hY, m.hX, P.ip, P.bc, Gw.data());
@@ -738,10 +749,11 @@ template <class T> static void solve_level(const SolveParams& P, const Mesh<T>& 
       const T sm = torch_sum3(a, b, c);
       rho_c[i] = (I1w[i] - sm) - I0[i];
     }
+    if (relax) z = u;
     for (int it = 0; it < P.max_iters; ++it) {
       const T sg = (T)sig[it], ta = (T)tau[it];
-      // dual step: p = prox_dual(p + sigma * D(grad z)), with z == u.
-      nabla_fwd(u.data(), 3, m.NZ, m.NY, m.NX, m.hZ, m.hY, m.hX, P.bc, zg.data());
+      // dual step: p = prox_dual(p + sigma * D(grad z)), with z == u unless relaxed.
+      nabla_fwd(relax ? z.data() : u.data(), 3, m.NZ, m.NY, m.NX, m.hZ, m.hY, m.hX, P.bc, zg.data());
       const T* dz = zg.data();
       if (P.use_aniso) { aniso_apply(zg.data(), s.data(), n.data(), t1.data(), t2.data(), N, Dz.data()); dz = Dz.data(); }
       for (size_t k = 0; k < 9 * N; ++k) { const T t = round_kernel_output(sg * dz[k]); dual[k] = p[k] + t; }
@@ -777,8 +789,14 @@ template <class T> static void solve_level(const SolveParams& P, const Mesh<T>& 
           st.du2 += d * d;
         }
         for (size_t i = 0; i < N; ++i) st.rho_abs += std::fabs((double)rho[i]);
-        store_stats(st, (double)lam, (double)wtv, stats + 3 * ((size_t)w * P.max_iters + it));
+        store_stats(st, (double)lam, (double)wtv, relax, stats + 3 * ((size_t)w * P.max_iters + it));
       }
+      if (relax)
+        for (size_t k = 0; k < 3 * N; ++k) {
+          const T d = un[k] - u[k];
+          const T e = round_kernel_output(theta * d);
+          z[k] = un[k] + e;
+        }
       u.swap(un);
     }
   }
@@ -1098,6 +1116,7 @@ using namespace orc;
     P.use_aniso = (int)params[10]; P.alpha = params[11]; P.beta = params[12];                                  \
     P.ip = (int)params[13]; P.bc = (int)params[14]; P.length_fixed = (int)params[15];                          \
     P.LZ = params[16]; P.LY = params[17]; P.LX = params[18]; P.huber_eps = params[19];                         \
+    P.relax_theta = params[20];                                                                                \
     solve<T>(P, I0, I1, NZ, NY, NX, u_in, p_in, u_out, p_out, stats);                                          \
   }                                                                                                            \
   extern "C" void orc_solve_##SFX(const double* params, const T* I0, const T* I1, int NZ, int NY, int NX,       \

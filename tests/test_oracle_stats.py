@@ -142,3 +142,43 @@ def test_huber_eps_zero_is_the_reference_step(O):
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     c = O.solve(I0, I1, O.solve_params(num_scales=2, max_warps=2, max_outer_iterations=4, huber_eps=0.05))
     assert not np.array_equal(a[0], c[0])
+
+
+def test_over_relaxation_second_dual_step(O):
+    """Over-relaxation (relax_theta, not in the reference): with constant images
+    (rho = 0, g = 0), no anisotropy and a ball too large to project onto, the
+    second dual step differs from the unrelaxed one by exactly sigma_1 grad of
+    theta (u^1 - u^0); the first iteration does not depend on theta."""
+    shape = (7, 8, 9)
+    I = np.full(shape, 0.5, np.float32)
+    r = np.random.default_rng(3)
+    zz, yy, xx = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij")
+    u0 = np.stack([np.sin(0.4 * xx + 0.1 * c) * np.cos(0.3 * yy) + 0.2 * zz / shape[0] for c in range(3)],
+                  -1).astype(np.float32)
+    p0 = np.zeros(shape + (3, 3), np.float32)
+    theta = 0.75
+    mk = lambda k, th: O.solve_params(num_scales=1, max_warps=1, max_outer_iterations=k, use_aniso=False,
+                                      weight_tv=1000.0, relax_theta=th)
+    u1, _ = O.solve(I, I, mk(1, theta), u=u0, p=p0)
+    u1_plain, _ = O.solve(I, I, mk(1, 0.0), u=u0, p=p0)
+    assert np.array_equal(u1, u1_plain)                      # iteration 0 uses z = u^0
+    _, p2r = O.solve(I, I, mk(2, theta), u=u0, p=p0)
+    _, p2 = O.solve(I, I, mk(2, 0.0), u=u0, p=p0)
+    sig, _ = O.step_schedule(mk(2, 0.0))
+    d = (u1.astype(np.float64) - u0.astype(np.float64))
+    want = sig[1] * theta * O.nabla_fwd(d, bc=1, dtype=np.float64)
+    got = p2r.astype(np.float64) - p2.astype(np.float64)
+    assert np.abs(want).max() > 1e-3
+    np.testing.assert_allclose(got, want, atol=2e-6 * np.abs(p2).max())
+
+
+def test_over_relaxation_restarts_each_warp(O):
+    """z is reset to u at every warp's first iteration: with one iteration per
+    warp theta changes nothing, over several warps and levels."""
+    I0, I1 = pair()
+    a = O.solve(I0, I1, O.solve_params(num_scales=2, max_warps=3, max_outer_iterations=1, relax_theta=0.9))
+    b = O.solve(I0, I1, O.solve_params(num_scales=2, max_warps=3, max_outer_iterations=1))
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    _, _, st = O.solve(I0, I1, O.solve_params(num_scales=1, max_warps=1, max_outer_iterations=3, relax_theta=0.5),
+                       stats=True)
+    assert np.all(np.isfinite(st[..., :2])) and np.all(np.isnan(st[..., 2]))
