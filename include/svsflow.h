@@ -210,6 +210,9 @@ typedef struct svs_tvl1_params {
   double huber_eps;           /* >= 0; 0 is the reference's TV.  eps > 0: Huber-TV, each dual row becomes
                                  v_c / max(1 + sigma_n eps / weight_tv, ||v_c|| / weight_tv) (no reference
                                  counterpart: tvl13d.py has no eps) */
+  double relax_theta;         /* 0 is the reference's z = u (tvl13d.py:251-252).  theta != 0: the dual step
+                                 differentiates z = u^n + theta (u^n - u^{n-1}), reset to u at each warp's first
+                                 iteration; the energy statistic is then NaN (no reference counterpart) */
 } svs_tvl1_params;
 
 /* Opaque solve plan: pyramid geometry, workspace layout, device tables and an

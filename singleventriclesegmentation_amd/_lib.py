@@ -38,6 +38,7 @@ class TVL1Params(ctypes.Structure):
         ("use_initial_state", _i),
         ("sweep_bx", _i), ("sweep_by", _i), ("sweep_cz", _i),
         ("huber_eps", ctypes.c_double),
+        ("relax_theta", ctypes.c_double),
     ]
 
 

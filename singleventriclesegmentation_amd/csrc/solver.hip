@@ -14,6 +14,7 @@
 //   RG         float4[N]           (rho_c, g_x, g_y, g_z) from the warp
 //   u[2]       float[3][N]  SoA    flow, ping-pong
 //   p[2]       float[9][N]  SoA    dual, p[3c+d] = row c, derivative d
+//   z[2]       float[3][N]  SoA    over-relaxed point (extended model only)
 // J, SN and RG are shared by the levels of an instance (levels run in turn).
 #include <algorithm>
 #include <array>
@@ -47,6 +48,7 @@ struct alignas(16) LevelDesc {
   float4* RG;
   float* u[2];
   float* p[2];
+  float* z[2];  // extended model only (see k_sweep's EXT): the dual step's point, ping-pong
   // Level 0 only: caller-bound inputs and outputs.
   const float* src0;
   const float* src1;
@@ -403,14 +405,20 @@ struct alignas(16) SweepTile {
 // per-iteration statistics terms (squared change of u, squared change of p,
 // |rho(u^n)|, and the l2 norms of the rows of D grad u^n) and writes them to
 // part[blockIdx.x] (see k_stats_reduce); u and p are computed as without.
-// HUBER: the dual rows use the Huber-TV step with factor hub (prox_dual_row)
-// and the energy the Huber function with eps; without, hub = 1 and eps = 0 are
-// compile-time constants, so the reference's step costs nothing extra.
-template <int BC, bool ANISO, bool UNIT_H, bool STATS, bool HUBER>
+// EXT, the extended model (neither part is in the reference): the dual rows
+// use the Huber-TV step with factor hub (prox_dual_row) and the energy the
+// Huber function with eps; the dual step differentiates the over-relaxed
+// point z^n = u^n + theta (u^n - u^{n-1}) instead of u^n.  z lives in its own
+// ping-pong field: the sweep stages z (u itself when zfirst, a warp's first
+// iteration) through LDS for the gradient, reads this column's u^n for the
+// primal step one plane ahead, and writes z' = u' + theta (u' - u^n) beside u'.
+// Without EXT, hub = 1, eps = 0 and z = u are compile-time facts, so the
+// reference's iteration costs nothing extra.
+template <int BC, bool ANISO, bool UNIT_H, bool STATS, bool EXT>
 __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ descs,
                                                 const SweepTile* __restrict__ tiles, int parity, float sigma,
                                                 float tau, float factor, float wtv, float hub, float eps,
-                                                float4* __restrict__ part) {
+                                                float theta, int zfirst, float4* __restrict__ part) {
   extern __shared__ float lds[];
   const SweepTile t = tiles[blockIdx.x];
   const LevelDesc& d = descs[t.inst];
@@ -442,6 +450,8 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   const uint32_t col = valid ? (uint32_t)(y * NX + x) : 0u;
   const uint32_t vo = col * 4u, vo4 = col * 16u;
   const svs_rsrc uin = make_rsrc(d.u[parity], 3u * F4), uout = make_rsrc(d.u[parity ^ 1], 3u * F4);
+  // The field the dual step differentiates (and stages with its ring).
+  const svs_rsrc zin = (EXT && !zfirst) ? make_rsrc(d.z[parity], 3u * F4) : uin;
   const svs_rsrc pin = make_rsrc(d.p[parity], 9u * F4), pout = make_rsrc(d.p[parity ^ 1], 9u * F4);
   const svs_rsrc SNr = make_rsrc(d.SN, 4u * F4), RGr = make_rsrc(d.RG, 4u * F4);
 
@@ -482,18 +492,22 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   for (int e = 0; e < 9; ++e) pc[e] = 0.0f;
   if (valid) {
     if (kstart >= 1)
-      for (int c = 0; c < 3; ++c) um[c] = bld(uin, vo, c * F4 + (kstart - 1) * P4);
-    for (int c = 0; c < 3; ++c) uc[c] = bld(uin, vo, c * F4 + kstart * P4);
+      for (int c = 0; c < 3; ++c) um[c] = bld(zin, vo, c * F4 + (kstart - 1) * P4);
+    for (int c = 0; c < 3; ++c) uc[c] = bld(zin, vo, c * F4 + kstart * P4);
     if (kstart + 1 < NZ)
-      for (int c = 0; c < 3; ++c) up[c] = bld(uin, vo, c * F4 + (kstart + 1) * P4);
+      for (int c = 0; c < 3; ++c) up[c] = bld(zin, vo, c * F4 + (kstart + 1) * P4);
     for (int e = 0; e < 9; ++e) pc[e] = bld(pin, vo, e * F4 + kstart * P4);
     if (ANISO) {
       const svs_f4 sn = bld4(SNr, vo4, 4u * kstart * P4);
       snc[0] = sn.x; snc[1] = sn.y; snc[2] = sn.z; snc[3] = sn.w;
     }
   }
-  for (int c = 0; c < 3; ++c) ringc[c] = bld(uin, ring_vo, c * F4 + kstart * P4);
+  for (int c = 0; c < 3; ++c) ringc[c] = bld(zin, ring_vo, c * F4 + kstart * P4);
   float r2a[3] = {0.0f, 0.0f, 0.0f}, r2b[3] = {0.0f, 0.0f, 0.0f}, r2c[3];
+  // EXT: this column's u^n of plane k-1 for the primal step (the staged field
+  // is z), loaded one step ahead.  The first step's primal plane is never in
+  // the chunk, so it starts unset.
+  float uoc[3] = {0.0f, 0.0f, 0.0f};
   float st_du = 0.0f, st_dp = 0.0f, st_rho = 0.0f, st_tv = 0.0f;  // STATS only
   // Each step reads this thread's u of the previous plane back from its own
   // slot of the other LDS buffer (only this thread writes that slot), so no
@@ -556,11 +570,11 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
           const float inc = sigma * dq[e];
           pv[e] = pc[3 * c + e] + inc;
         }
-        prox_dual_row(pv, wtv, HUBER ? hub : 1.0f);
+        prox_dual_row(pv, wtv, EXT ? hub : 1.0f);
         if (wtv == 0.0f) { pv[0] = 0.0f; pv[1] = 0.0f; pv[2] = 0.0f; }  // tvl13d.py:223-224
         if (STATS && store_p) {
           st_tv += huber_of_norm(__builtin_amdgcn_sqrtf(dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2]),
-                                 HUBER ? eps : 0.0f);
+                                 EXT ? eps : 0.0f);
           for (int e = 0; e < 3; ++e) {
             const float dd = pv[e] - pc[3 * c + e];
             st_dp += dd * dd;
@@ -585,15 +599,19 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
       }
     }
 
-    for (int c = 0; c < 3; ++c) un[c] = bld(uin, vo, c * F4 + k2 * P4);
-    for (int c = 0; c < 3; ++c) ringn[c] = bld(uin, ring_vo, c * F4 + k1 * P4);
+    for (int c = 0; c < 3; ++c) un[c] = bld(zin, vo, c * F4 + k2 * P4);
+    for (int c = 0; c < 3; ++c) ringn[c] = bld(zin, ring_vo, c * F4 + k1 * P4);
     for (int e = 0; e < 9; ++e) pn[e] = bld(pin, vo, e * F4 + k1 * P4);
+    float uon[3];
+    if (EXT)
+      for (int c = 0; c < 3; ++c) uon[c] = bld(uin, vo, c * F4 + (uint32_t)min(k, NZ - 1) * P4);
 
     // ---- primal step at plane k-1 on the tile interior
     const int km = k - 1;
     if (interior && km >= z0 && km < z1) {
       const float* Rp = R + (km & 1) * 6 * RPL;
       const float g[3] = {rgc[1], rgc[2], rgc[3]};
+      const float* uold = EXT ? uoc : um;  // u^n of this column at plane k-1
       float v[3];
       for (int c = 0; c < 3; ++c) {
         const float* Rx = Rp + (2 * c) * RPL;
@@ -603,19 +621,27 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
         const float az = cd_adj(r2a[c], r2b[c], r2c[c], km, NZ, BC);
         const float div = ax / hX + ay / hY + az / hZ;
         const float step = tau * div;
-        v[c] = um[c] - step;
+        v[c] = uold[c] - step;
       }
-      const float sm = (um[0] * g[0] + um[2] * g[2]) + um[1] * g[1];  // (a + c) + b, as torch's sum
+      const float sm = (uold[0] * g[0] + uold[2] * g[2]) + uold[1] * g[1];  // (a + c) + b, as torch's sum
       const float rho = rgc[0] + sm;
       prox_primal_point(v, rho, g, factor);
       if (STATS) {
         st_rho += fabsf(rho);
         for (int c = 0; c < 3; ++c) {
-          const float dd = v[c] - um[c];
+          const float dd = v[c] - uold[c];
           st_du += dd * dd;
         }
       }
       for (int c = 0; c < 3; ++c) bst(uout, vo, c * F4 + km * P4, v[c]);
+      if (EXT) {
+        const svs_rsrc zout = make_rsrc(d.z[parity ^ 1], 3u * F4);
+        for (int c = 0; c < 3; ++c) {
+          const float dd = v[c] - uold[c];
+          const float ee = theta * dd;
+          bst(zout, vo, c * F4 + km * P4, v[c] + ee);
+        }
+      }
     }
     if (ANISO) {
       const svs_f4 sn = bld4(SNr, vo4, 4u * k1 * P4);
@@ -632,6 +658,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
       uc[c] = up[c];
       up[c] = un[c];
       ringc[c] = ringn[c];
+      if (EXT) uoc[c] = uon[c];
     }
     for (int e = 0; e < 9; ++e) pc[e] = pn[e];
     for (int e = 0; e < 4; ++e) { snc[e] = snn[e]; rgc[e] = rgn[e]; }
@@ -679,11 +706,15 @@ constexpr int kResidentPerThread = 2;
 // part (null: off): per iteration it and wave, the four statistics sums of
 // the wave's voxels at part[(it * gridDim.x + blockIdx.x) * waves + wave] (see
 // k_sweep's STATS and k_stats_reduce).
+// theta (not in the reference, 0 there): the dual step differentiates the
+// over-relaxed point z = u + theta (u - u_prev), kept in registers beside u and
+// equal to u at the launch's first iteration (a warp's start); theta = 0 keeps
+// z = u without arithmetic.
 template <int BC, bool ANISO, bool UNIT_H, int V>
 __global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __restrict__ descs,
                                                          const float* __restrict__ sched, int first, int K,
                                                          int sched_len, int parity, float wtv, int wtv_zero,
-                                                         float eps, float4* __restrict__ part) {
+                                                         float eps, float theta, float4* __restrict__ part) {
   extern __shared__ float lds[];
   const LevelDesc& d = descs[blockIdx.x];
   const int NZ = d.NZ, NY = d.NY, NX = d.NX;
@@ -696,7 +727,7 @@ __global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __rest
   const float* pin = d.p[parity];
 
   constexpr bool KEEP = V == 1;  // (s, n) and (rho_c, g) held in registers
-  float u[V][3], p[V][9], sn[KEEP ? V : 1][4], rg[KEEP ? V : 1][4];
+  float u[V][3], zr[V][3], p[V][9], sn[KEEP ? V : 1][4], rg[KEEP ? V : 1][4];
   int vz[V], vy[V], vx[V];
   bool own[V];
 #pragma unroll
@@ -708,7 +739,10 @@ __global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __rest
     vy[j] = (vv / NX) % NY;
     vz[j] = vv / PL;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) u[j][c] = own[j] ? uin[(int64_t)c * N + vv] : 0.0f;
+    for (int c = 0; c < 3; ++c) {
+      u[j][c] = own[j] ? uin[(int64_t)c * N + vv] : 0.0f;
+      zr[j][c] = u[j][c];
+    }
 #pragma unroll
     for (int e = 0; e < 9; ++e) p[j][e] = own[j] ? pin[(int64_t)e * N + vv] : 0.0f;
     if (KEEP) {
@@ -728,7 +762,7 @@ __global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __rest
       if (own[j]) {
         const int v = (int)threadIdx.x + j * (int)blockDim.x;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) U[c * N + v] = u[j][c];
+        for (int c = 0; c < 3; ++c) U[c * N + v] = zr[j][c];
       }
     __syncthreads();
     // ---- dual step
@@ -831,7 +865,16 @@ __global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __rest
           }
         }
 #pragma unroll
-        for (int c = 0; c < 3; ++c) u[j][c] = nv[c];
+        for (int c = 0; c < 3; ++c) {
+          if (theta != 0.0f) {
+            const float dd = nv[c] - u[j][c];
+            const float ee = theta * dd;
+            zr[j][c] = nv[c] + ee;
+          } else {
+            zr[j][c] = nv[c];
+          }
+          u[j][c] = nv[c];
+        }
       }
     if (part) {  // uniform: every thread of the work-group takes it
       const float4 w = make_float4(wave_sum(st_du), wave_sum(st_dp), wave_sum(st_rho), wave_sum(st_tv));
@@ -861,7 +904,9 @@ __global__ __launch_bounds__(1024) void k_sweep_resident(const LevelDesc* __rest
 // Work-group (n, k) of K * ninst adds instance k's entries in a fixed order, in
 // double, and writes out[(n * ninst + k) * 3 + j]: j = 0 ||u^{n+1} - u^n||,
 // j = 1 ||p^{n+1} - p^n|| (the residual norms of tvl13d.py:222, 237, 253),
-// j = 2 the energy lambda sum|rho(u^n)| + lambda_tv sum_c ||D grad u^n_c||.
+// j = 2 the energy lambda sum|rho(u^n)| + lambda_tv sum_c ||D grad u^n_c|| (the
+// caller passes lam = NaN when the dual step differentiates an over-relaxed
+// point instead of u^n: the energy is then not available).
 // Instance k's work-groups are csr_idx[csr_off[k] .. csr_off[k + 1]), or work-
 // group k alone when csr_off is null (the resident sweep).
 __global__ __launch_bounds__(64) void k_stats_reduce(const float4* __restrict__ part, const int* __restrict__ csr_off,
@@ -1140,6 +1185,10 @@ static void choose_sweep_shape(const svs_tvl1_params& P, LevelPlan& L, int ninst
 
 using namespace svs;
 
+// The extended model (Huber-TV, over-relaxation; no reference counterpart):
+// its sweeps keep the over-relaxed point z in fields of its own.
+static bool extended_model(const svs_tvl1_params& P) { return P.huber_eps > 0.0 || P.relax_theta != 0.0; }
+
 // Workspace layout, computed identically by the sizing and the plan builder.
 static int plan_layout(svs_plan* pl, char* base, size_t* total_out) {
   const svs_tvl1_params& P = pl->P;
@@ -1175,6 +1224,7 @@ static int plan_layout(svs_plan* pl, char* base, size_t* total_out) {
       for (int b = 0; b < 2; ++b) {
         d.u[b] = (float*)take(12 * N);
         d.p[b] = (float*)take(36 * N);
+        if (extended_model(P)) d.z[b] = (float*)take(12 * N);
       }
     }
   }
@@ -1243,6 +1293,7 @@ static int validate_params(const svs_tvl1_params* P) {
   SVS_REQUIRE(!P->use_blur || (P->blur_K >= 1 && (P->blur_K & 1) && P->blur_weights),
               "blur enabled but kernel size/weights invalid");
   SVS_REQUIRE(P->huber_eps >= 0.0 && std::isfinite(P->huber_eps), "huber_eps must be finite and >= 0");
+  SVS_REQUIRE(std::isfinite(P->relax_theta), "relax_theta must be finite");
   return 0;
 }
 
@@ -1473,29 +1524,31 @@ static int upload_tiles(svs_plan* pl, int s, hipStream_t st) {
   return 0;
 }
 
-// The scalars of one sweep: schedule position n's steps and Huber factor.
+// The scalars of one sweep: schedule position n's steps, and for the extended
+// model its Huber factor and eps, theta and whether z is still u (n == 0).
 struct SweepScalars {
-  float sg, ta, fac, wtv, hub, eps;
+  float sg, ta, fac, wtv, hub, eps, theta;
+  int zfirst;
+  bool ext;
 };
 
-template <int BC, bool ANISO, bool STATS, bool HUBER>
+template <int BC, bool ANISO, bool STATS, bool EXT>
 static void launch_sweep_s(const LevelPlan& L, int parity, const SweepScalars& a, float4* part, hipStream_t st) {
   if (L.unit_h)
-    hipLaunchKernelGGL((k_sweep<BC, ANISO, true, STATS, HUBER>), dim3(L.ntiles), dim3(L.threads), L.lds, st, L.dev,
-                       L.tiles, parity, a.sg, a.ta, a.fac, a.wtv, a.hub, a.eps, part);
+    hipLaunchKernelGGL((k_sweep<BC, ANISO, true, STATS, EXT>), dim3(L.ntiles), dim3(L.threads), L.lds, st, L.dev,
+                       L.tiles, parity, a.sg, a.ta, a.fac, a.wtv, a.hub, a.eps, a.theta, a.zfirst, part);
   else
-    hipLaunchKernelGGL((k_sweep<BC, ANISO, false, STATS, HUBER>), dim3(L.ntiles), dim3(L.threads), L.lds, st,
-                       L.dev, L.tiles, parity, a.sg, a.ta, a.fac, a.wtv, a.hub, a.eps, part);
+    hipLaunchKernelGGL((k_sweep<BC, ANISO, false, STATS, EXT>), dim3(L.ntiles), dim3(L.threads), L.lds, st,
+                       L.dev, L.tiles, parity, a.sg, a.ta, a.fac, a.wtv, a.hub, a.eps, a.theta, a.zfirst, part);
 }
 
 template <int BC, bool ANISO>
 static void launch_sweep_t(const LevelPlan& L, int parity, const SweepScalars& a, float4* part, hipStream_t st) {
-  const bool hub = a.eps > 0.0f;
   if (part) {
-    if (hub) launch_sweep_s<BC, ANISO, true, true>(L, parity, a, part, st);
+    if (a.ext) launch_sweep_s<BC, ANISO, true, true>(L, parity, a, part, st);
     else launch_sweep_s<BC, ANISO, true, false>(L, parity, a, part, st);
   } else {
-    if (hub) launch_sweep_s<BC, ANISO, false, true>(L, parity, a, nullptr, st);
+    if (a.ext) launch_sweep_s<BC, ANISO, false, true>(L, parity, a, nullptr, st);
     else launch_sweep_s<BC, ANISO, false, false>(L, parity, a, nullptr, st);
   }
 }
@@ -1505,7 +1558,8 @@ static void launch_sweep_t(const LevelPlan& L, int parity, const SweepScalars& a
 static void launch_sweep(const svs_plan* pl, const LevelPlan& L, int parity, int n, hipStream_t st,
                          float4* part = nullptr) {
   const svs_tvl1_params& P = pl->P;
-  const SweepScalars a{pl->sig[n], pl->tau[n], pl->factor[n], P.weight_tv, pl->hub[n], (float)P.huber_eps};
+  const SweepScalars a{pl->sig[n], pl->tau[n], pl->factor[n], P.weight_tv, pl->hub[n], (float)P.huber_eps,
+                       (float)P.relax_theta, n == 0 ? 1 : 0, extended_model(P)};
   const bool an = P.use_aniso != 0;
   switch (P.boundary * 2 + (an ? 1 : 0)) {
     case 0: launch_sweep_t<BD_NEAREST, false>(L, parity, a, part, st); break;
@@ -1524,13 +1578,13 @@ static void launch_resident_v(const svs_plan* pl, const LevelPlan& L, int parity
   const float wtv = P.weight_tv;
   const int wz = P.weight_tv == 0.0f;
   const int len = P.max_outer_iterations;
-  const float eps = (float)P.huber_eps;
+  const float eps = (float)P.huber_eps, theta = (float)P.relax_theta;
   if (L.unit_h)
     hipLaunchKernelGGL((k_sweep_resident<BC, ANISO, true, V>), dim3(pl->ninst), dim3(L.res_threads), L.res_lds, st,
-                       L.dev, pl->sched_dev, first, K, len, parity, wtv, wz, eps, part);
+                       L.dev, pl->sched_dev, first, K, len, parity, wtv, wz, eps, theta, part);
   else
     hipLaunchKernelGGL((k_sweep_resident<BC, ANISO, false, V>), dim3(pl->ninst), dim3(L.res_threads), L.res_lds,
-                       st, L.dev, pl->sched_dev, first, K, len, parity, wtv, wz, eps, part);
+                       st, L.dev, pl->sched_dev, first, K, len, parity, wtv, wz, eps, theta, part);
 }
 
 template <int BC, bool ANISO>
@@ -1566,21 +1620,21 @@ static int allow_resident_lds() {
   return 0;
 }
 
-template <bool STATS, bool HUBER>
+template <bool STATS, bool EXT>
 static int allow_sweep_lds(int v) {
   const void* kernels[] = {
-      (const void*)k_sweep<BD_NEAREST, false, false, STATS, HUBER>,
-      (const void*)k_sweep<BD_NEAREST, true, false, STATS, HUBER>,
-      (const void*)k_sweep<BD_MIRROR, false, false, STATS, HUBER>,
-      (const void*)k_sweep<BD_MIRROR, true, false, STATS, HUBER>,
-      (const void*)k_sweep<BD_REFLECT, false, false, STATS, HUBER>,
-      (const void*)k_sweep<BD_REFLECT, true, false, STATS, HUBER>,
-      (const void*)k_sweep<BD_NEAREST, false, true, STATS, HUBER>,
-      (const void*)k_sweep<BD_NEAREST, true, true, STATS, HUBER>,
-      (const void*)k_sweep<BD_MIRROR, false, true, STATS, HUBER>,
-      (const void*)k_sweep<BD_MIRROR, true, true, STATS, HUBER>,
-      (const void*)k_sweep<BD_REFLECT, false, true, STATS, HUBER>,
-      (const void*)k_sweep<BD_REFLECT, true, true, STATS, HUBER>};
+      (const void*)k_sweep<BD_NEAREST, false, false, STATS, EXT>,
+      (const void*)k_sweep<BD_NEAREST, true, false, STATS, EXT>,
+      (const void*)k_sweep<BD_MIRROR, false, false, STATS, EXT>,
+      (const void*)k_sweep<BD_MIRROR, true, false, STATS, EXT>,
+      (const void*)k_sweep<BD_REFLECT, false, false, STATS, EXT>,
+      (const void*)k_sweep<BD_REFLECT, true, false, STATS, EXT>,
+      (const void*)k_sweep<BD_NEAREST, false, true, STATS, EXT>,
+      (const void*)k_sweep<BD_NEAREST, true, true, STATS, EXT>,
+      (const void*)k_sweep<BD_MIRROR, false, true, STATS, EXT>,
+      (const void*)k_sweep<BD_MIRROR, true, true, STATS, EXT>,
+      (const void*)k_sweep<BD_REFLECT, false, true, STATS, EXT>,
+      (const void*)k_sweep<BD_REFLECT, true, true, STATS, EXT>};
   for (const void* k : kernels) SVS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, v));
   return 0;
 }
@@ -1663,6 +1717,7 @@ static int enqueue_solve(svs_plan* pl, hipStream_t st) {
     }
     parity = 0;
     const int K = P.max_outer_iterations;
+    const double elam = P.relax_theta != 0.0 ? std::nan("") : (double)P.weight_matching;
     for (int w = 0; w < P.max_warps; ++w) {
       k_linearise<<<L.nblocks, kBlock, 0, st>>>(L.dev, L.block_start, n, parity, ip, bc);
       SVS_LAUNCHED();
@@ -1674,7 +1729,7 @@ static int enqueue_solve(svs_plan* pl, hipStream_t st) {
         SVS_LAUNCHED();
         if (sout) {
           k_stats_reduce<<<K * n, 64, 0, st>>>(part, nullptr, nullptr, n, L.res_threads / 64, n,
-                                               (double)P.weight_matching, (double)P.weight_tv, sout);
+                                               elam, (double)P.weight_tv, sout);
           SVS_LAUNCHED();
         }
         parity ^= K & 1;
@@ -1687,7 +1742,7 @@ static int enqueue_solve(svs_plan* pl, hipStream_t st) {
       }
       if (sout && K > 0) {
         k_stats_reduce<<<K * n, 64, 0, st>>>(part, L.tile_csr, L.tile_csr + n + 1, L.ntiles, 1, n,
-                                             (double)P.weight_matching, (double)P.weight_tv, sout);
+                                             elam, (double)P.weight_tv, sout);
         SVS_LAUNCHED();
       }
     }

@@ -44,6 +44,9 @@ class SolverConfig:
     # Huber-TV smoothing (not in the reference, whose TV is eps = 0): each dual
     # row becomes v_c / max(1 + sigma_n eps / weight_tv, ||v_c|| / weight_tv).
     huber_eps: float = 0.0
+    # Over-relaxation (not in the reference, whose z = u): the dual step
+    # differentiates z = u^n + theta (u^n - u^{n-1}), reset to u every warp.
+    relax_theta: float = 0.0
 
     def native(self, use_initial_state=False):
         p = TVL1Params()
@@ -70,6 +73,7 @@ class SolverConfig:
         p.use_initial_state = 1 if use_initial_state else 0
         p.sweep_bx, p.sweep_by, p.sweep_cz = (int(v) for v in self.sweep_shape)
         p.huber_eps = float(self.huber_eps)
+        p.relax_theta = float(self.relax_theta)
         return p, weights  # keep `weights` alive while `p` is used
 
     def level_shapes(self, shape):

@@ -49,6 +49,7 @@ class TVL13DOpticalFlow:
         autotune: bool = False,
         record_stats: bool = False,
         huber_eps: float = 0.0,
+        relax_theta: float = 0.0,
     ):
         """Arguments as in tvl13d.py:14-52; ``length`` is (lx, ly, lz).
 
@@ -67,7 +68,11 @@ class TVL13DOpticalFlow:
         Huber-TV one (the reference's TV is eps = 0): each dual row becomes
         v_c / max(1 + sigma_n eps / weight_tv, ||v_c|| / weight_tv), and the
         energy uses the Huber function of the row norms.  compute_unfused
-        follows the reference's operators and supports eps = 0 only."""
+        follows the reference's operators and supports eps = 0 only.
+        ``relax_theta`` != 0 over-relaxes the point the dual step
+        differentiates, z = u^n + theta (u^n - u^{n-1}) (the reference keeps
+        z = u, tvl13d.py:251-252), restarting from z = u at every warp like the
+        step sizes; the energy statistic is then NaN."""
         self.num_scales = num_scales
         self.max_warps = max_warps
         self.max_outer_iterations = max_outer_iterations
@@ -92,6 +97,9 @@ class TVL13DOpticalFlow:
         if not huber_eps >= 0.0:
             raise ValueError("huber_eps must be >= 0")
         self.huber_eps = float(huber_eps)
+        if not math.isfinite(relax_theta):
+            raise ValueError("relax_theta must be finite")
+        self.relax_theta = float(relax_theta)
         self.config = SolverConfig(
             num_scales=num_scales, max_warps=max_warps, max_outer_iterations=max_outer_iterations,
             weight_matching=weight_matching, weight_tv=weight_tv, pd_type=primaldual_algorithm_type,
@@ -99,7 +107,8 @@ class TVL13DOpticalFlow:
             aniso_params=tuple(float(v) for v in anisotropic_diff_params), use_blur=bool(use_gaussian_blur),
             blur_sigma=float(gaussian_blur_sigma), interpolation=int(self.interpolation_type),
             boundary=int(self.boundary_type), length_fixed=(length_type == "fixed"),
-            length=(float(self.length_x), float(self.length_y), float(self.length_z)), huber_eps=self.huber_eps)
+            length=(float(self.length_x), float(self.length_y), float(self.length_z)), huber_eps=self.huber_eps,
+            relax_theta=self.relax_theta)
         self._plans = {}
 
     # ------------------------------------------------------------ fused path
@@ -206,8 +215,9 @@ class TVL13DOpticalFlow:
         if self.use_anisotropic_diff:
             aniso = opticalFlow.AnisotropicNabla3D(meshInfo, self.anisotropic_diff_alpha, self.anistropic_diff_beta)
             tangents = aniso.computeTangentVecs(I1_grad)
-        z = u
+        theta = float(np.float32(self.relax_theta))
         for w in range(self.max_warps):
+            z = u  # tvl13d.py:194; with over-relaxation, every warp restarts from z = u
             I1_warped = warping.forward(I1, u)
             I1_warped_grad = warping.forwardVectorField(I1_grad, u)
             rho_c = I1_warped - torch.sum(u * I1_warped_grad, dim=3) - I0
@@ -227,12 +237,13 @@ class TVL13DOpticalFlow:
                 if rec is not None:
                     rec[s, w, n, 0] = torch.norm(u - uold)
                     rec[s, w, n, 1] = torch.norm(p - pold)
-                    rec[s, w, n, 2] = (float(np.float32(self.weight_matching)) * rho.abs().sum()
+                    rec[s, w, n, 2] = (float("nan") if theta != 0.0 else
+                                       float(np.float32(self.weight_matching)) * rho.abs().sum()
                                        + float(np.float32(self.weight_tv))
                                        * torch.linalg.vector_norm(Dz_grad, dim=-1).sum())
                 if self.primaldual_algorithm_type == 2:
-                    theta = 1. / math.sqrt(1. + 2. * gamma * tau)
-                    tau *= theta
-                    sigma /= theta
-                z = u
+                    th = 1. / math.sqrt(1. + 2. * gamma * tau)
+                    tau *= th
+                    sigma /= th
+                z = u if theta == 0.0 else u + theta * (u - uold)
         return u, p

@@ -172,3 +172,62 @@ def test_stats_follow_shape_changes_and_toggling(O, dev):
     torch.cuda.synchronize()
     np.testing.assert_allclose(plan.stats.cpu().numpy()[:, :, :, 0], so, rtol=RTOL)
     plan.close()
+
+
+@pytest.mark.parametrize("boundary,aniso,theta,eps", [("MIRROR", True, 0.5, 0.0), ("NEAREST", False, 1.0, 0.0),
+                                                      ("REFLECT", True, 0.8, 0.05)])
+def test_over_relaxation_matches_oracle(O, dev, boundary, aniso, theta, eps):
+    """Over-relaxation (relax_theta != 0, not in the reference), alone and with
+    Huber-TV: the fused solve through a marching and a resident level equals
+    the oracle's bits, the residual statistics agree and the energy is NaN,
+    and TVL13DOpticalFlow(relax_theta=...) returns the plan's bits."""
+    I0, I1, _ = pair((20, 24, 28), seed=17)
+    S, W, K = 2, 3, 8
+    w = gaussian_weights3d(1.0)
+    prm = O.solve_params(num_scales=S, max_warps=W, max_outer_iterations=K, boundary=boundary, use_aniso=aniso,
+                         huber_eps=eps, relax_theta=theta)
+    uo, po, so = O.solve(O.blur(I0, w), O.blur(I1, w), prm, stats=True)
+    cfg = SolverConfig(num_scales=S, max_warps=W, max_outer_iterations=K,
+                       boundary={"NEAREST": 0, "MIRROR": 1, "REFLECT": 2}[boundary], use_aniso=aniso,
+                       huber_eps=eps, relax_theta=theta)
+    plan = Plan(cfg, [I0.shape], device=dev)
+    a, b = torch.from_numpy(I0).to(dev), torch.from_numpy(I1).to(dev)
+    u = torch.empty(I0.shape + (3,), device=dev)
+    p = torch.empty(I0.shape + (3, 3), device=dev)
+    plan.bind(0, a, b, u_out=u, p_out=p)
+    plan.set_stats(True)
+    plan.run()
+    torch.cuda.synchronize()
+    assert plan.info(1)["resident"] > 0
+    assert np.array_equal(u.cpu().numpy(), uo) and np.array_equal(p.cpu().numpy(), po)
+    st = plan.stats.cpu().numpy().astype(np.float64)[:, :, :, 0]
+    np.testing.assert_allclose(st[..., :2], so[..., :2], rtol=RTOL)
+    assert np.all(np.isnan(st[..., 2])) and np.all(np.isnan(so[..., 2]))
+    plan.close()
+    solver = make_solver(S, W, K, boundary_type=boundary, use_anisotropic_diff=aniso, huber_eps=eps,
+                         relax_theta=theta)
+    uc, pc = solver.compute(a, b, None, None)
+    assert np.array_equal(uc.cpu().numpy(), uo) and np.array_equal(pc.cpu().numpy(), po)
+    # the operator-by-operator driver with the same relaxation (Huber has no
+    # reference operator, so only eps = 0 runs there)
+    if eps == 0.0:
+        ur, pr = solver.compute_unfused(a, b, None, None)
+        assert torch.equal(ur, uc) and torch.equal(pr, pc)
+
+
+def test_over_relaxation_marching_equals_resident(dev):
+    """The 10^3 level with theta != 0 gives the same bits resident or marching."""
+    I0, I1, _ = pair((20, 20, 20), seed=19)
+    outs = []
+    for mode in (1, 0):
+        cfg = SolverConfig(num_scales=2, max_warps=2, max_outer_iterations=6, relax_theta=0.7)
+        plan = Plan(cfg, [I0.shape], device=dev)
+        a, b = torch.from_numpy(I0).to(dev), torch.from_numpy(I1).to(dev)
+        u = torch.empty(I0.shape + (3,), device=dev)
+        plan.bind(0, a, b, u_out=u)
+        plan.set_resident(1, mode)
+        plan.run()
+        torch.cuda.synchronize()
+        outs.append(u.cpu().numpy())
+        plan.close()
+    assert np.array_equal(outs[0], outs[1])
