@@ -483,8 +483,8 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   const int kstart = z0 > 0 ? z0 - 1 : 0;
   const int kend = z1 < NZ ? z1 : NZ;  // the step k == NZ only finishes plane NZ-1
 
-  // Registers: own-column u of planes k-1..k+1 (k+2 in flight; k-1 is
-  // refreshed from LDS each step), the dual and anisotropy of plane k (k+1 in
+  // Registers: own-column u (z in the extended model) of planes k-1..k+1 (k+2
+  // in flight; k-1 is refreshed from LDS each step), the dual and anisotropy of plane k (k+1 in
   // flight), rho_c/g of plane k-1 (k in flight), the ring value of plane k
   // (k+1 in flight) and the z-derivative column of r = D p' for planes k-2..k.
   float um[3] = {0.0f, 0.0f, 0.0f}, uc[3] = {0.0f, 0.0f, 0.0f}, up[3] = {0.0f, 0.0f, 0.0f};
