@@ -463,6 +463,9 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   // then need no branch (a divergent one around them made the compiler merge
   // their wave offsets into VGPRs and read them back lane by lane).
   const bool ring = tid < nring;
+  // Waves with no ring lane skip the ring loads altogether (a wave-uniform
+  // branch: no divergent join for the compiler to merge the offsets across).
+  const bool ring_wave = __builtin_amdgcn_readfirstlane(tid >> 6) * 64 < nring;
   int ring_lds = 0;
   uint32_t ring_vo = kOutside;
   if (ring) {
@@ -502,7 +505,8 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
       snc[0] = sn.x; snc[1] = sn.y; snc[2] = sn.z; snc[3] = sn.w;
     }
   }
-  for (int c = 0; c < 3; ++c) ringc[c] = bld(zin, ring_vo, c * F4 + kstart * P4);
+  if (ring_wave)
+    for (int c = 0; c < 3; ++c) ringc[c] = bld(zin, ring_vo, c * F4 + kstart * P4);
   float r2a[3] = {0.0f, 0.0f, 0.0f}, r2b[3] = {0.0f, 0.0f, 0.0f}, r2c[3];
   // EXT: this column's u^n of plane k-1 for the primal step (the staged field
   // is z), loaded one step ahead.  The first step's primal plane is never in
@@ -600,7 +604,9 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
     }
 
     for (int c = 0; c < 3; ++c) un[c] = bld(zin, vo, c * F4 + k2 * P4);
-    for (int c = 0; c < 3; ++c) ringn[c] = bld(zin, ring_vo, c * F4 + k1 * P4);
+    for (int c = 0; c < 3; ++c) ringn[c] = 0.0f;
+    if (ring_wave)
+      for (int c = 0; c < 3; ++c) ringn[c] = bld(zin, ring_vo, c * F4 + k1 * P4);
     for (int e = 0; e < 9; ++e) pn[e] = bld(pin, vo, e * F4 + k1 * P4);
     float uon[3];
     if (EXT)
