@@ -458,13 +458,13 @@ __global__ __launch_bounds__(1024) void k_sweep(const LevelDesc* __restrict__ de
   // This thread's position on the outer ring of the u tile (one per thread;
   // the host guarantees 2*UW + 2*BY <= BX*BY).
   const int nring = 2 * UW + 2 * BY;
-  // Lanes off the ring, or with a ring position outside the volume, load
-  // from past the buffer and get 0 without a memory access: the ring loads
-  // then need no branch (a divergent one around them made the compiler merge
-  // their wave offsets into VGPRs and read them back lane by lane).
+  // Only waves holding ring lanes issue the ring loads, behind a wave-uniform
+  // branch (a divergent one made the compiler merge the loads' wave offsets
+  // into VGPRs and read them back lane by lane; loads issued by every wave
+  // cost the sweep 3 % in address processing).  Within those waves, lanes off
+  // the ring, or with a ring position outside the volume, load from past the
+  // buffer and get 0 without a memory access.
   const bool ring = tid < nring;
-  // Waves with no ring lane skip the ring loads altogether (a wave-uniform
-  // branch: no divergent join for the compiler to merge the offsets across).
   const bool ring_wave = __builtin_amdgcn_readfirstlane(tid >> 6) * 64 < nring;
   int ring_lds = 0;
   uint32_t ring_vo = kOutside;
