@@ -231,3 +231,30 @@ def test_over_relaxation_marching_equals_resident(dev):
         outs.append(u.cpu().numpy())
         plan.close()
     assert np.array_equal(outs[0], outs[1])
+
+
+def test_extended_model_and_stats_on_a_fixed_mesh(O, dev):
+    """Non-unit spacing (fixed lengths: h != 1, the kernels' general-h variant)
+    with Huber-TV, over-relaxation and statistics together: bits and residuals
+    as the oracle's."""
+    I0, I1, _ = pair((18, 20, 22), seed=23)
+    S, W, K = 2, 2, 6
+    length = (1.3, 0.9, 1.1)   # (lx, ly, lz)
+    w = gaussian_weights3d(1.0)
+    prm = O.solve_params(num_scales=S, max_warps=W, max_outer_iterations=K, length_type="fixed", length=length,
+                         huber_eps=0.03, relax_theta=0.6)
+    uo, po, so = O.solve(O.blur(I0, w), O.blur(I1, w), prm, stats=True)
+    cfg = SolverConfig(num_scales=S, max_warps=W, max_outer_iterations=K, length_fixed=True, length=length,
+                       huber_eps=0.03, relax_theta=0.6)
+    plan = Plan(cfg, [I0.shape], device=dev)
+    a, b = torch.from_numpy(I0).to(dev), torch.from_numpy(I1).to(dev)
+    u = torch.empty(I0.shape + (3,), device=dev)
+    p = torch.empty(I0.shape + (3, 3), device=dev)
+    plan.bind(0, a, b, u_out=u, p_out=p)
+    plan.set_stats(True)
+    plan.run()
+    torch.cuda.synchronize()
+    assert np.array_equal(u.cpu().numpy(), uo) and np.array_equal(p.cpu().numpy(), po)
+    st = plan.stats.cpu().numpy().astype(np.float64)[:, :, :, 0]
+    np.testing.assert_allclose(st[..., :2], so[..., :2], rtol=RTOL)
+    plan.close()
