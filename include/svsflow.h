@@ -309,7 +309,9 @@ int svs_tvl1_plan_set_resident(svs_plan* plan, int level, int mode, void* stream
  * stay allocated while statistics are on; out = NULL turns them off.  The plan
  * allocates its own buffer for the partial sums (freed when statistics are
  * turned off or the plan is destroyed) and re-captures its graph at the next
- * run.  No reference counterpart beyond the discarded norms. */
+ * run.  Replacing or removing an output waits for the plan's enqueued work,
+ * so the previous output may be freed on return.  No reference counterpart
+ * beyond the discarded norms. */
 int svs_tvl1_plan_set_stats(svs_plan* plan, float* out, void* stream);
 
 /* One frame pair without a graph: TVL13DOpticalFlow.compute(I0, I1, u, p).

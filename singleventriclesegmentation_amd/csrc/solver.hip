@@ -2056,7 +2056,12 @@ int svs_tvl1_plan_set_stats(svs_plan* pl, float* out, void* stream) {
   SVS_REQUIRE(groups < ((int64_t)1 << 31), "max_outer_iterations x instances too large for statistics");
   hipStream_t st = (hipStream_t)stream;
   if (int rc = after_last_work(pl, st)) return rc;
-  if (out != pl->stats_out) drop_graph(pl);
+  if (out != pl->stats_out) {
+    // The previous output may still be written by enqueued runs (on any
+    // stream): wait for them, so that the caller may free it on return.
+    if (pl->stats_out) SVS_HIP(hipEventSynchronize(pl->last_work));
+    drop_graph(pl);
+  }
   pl->stats_out = out;
   if (!out && pl->stats_part) {  // off: release the partial sums once no work reads them
     SVS_HIP(hipEventSynchronize(pl->last_work));
