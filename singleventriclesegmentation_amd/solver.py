@@ -203,8 +203,10 @@ class Plan:
         fills in stream order: [..., 0] = ||u^{n+1} - u^n||, [..., 1] =
         ||p^{n+1} - p^n|| (the residual norms tvl13d.py:222, 237, 253 computes
         and discards), [..., 2] = the energy lambda sum|rho(u^n)| + lambda_tv
-        sum_c ||D grad u^n_c|| of the iterate entering iteration n.  The values
-        are the same on every run; u and p do not change."""
+        sum_c ||D grad u^n_c|| of the iterate entering iteration n (NaN with
+        over-relaxation).  The values are the same on every run; u and p do not
+        change.  Replacing or dropping the tensor waits for the plan's enqueued
+        runs."""
         c = self.cfg
         out = None
         if enabled:

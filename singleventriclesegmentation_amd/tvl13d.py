@@ -198,6 +198,7 @@ class TVL13DOpticalFlow:
             us[s - 1][..., 1] *= meshes[s - 1].getLY() / meshes[s].getLY()
             us[s - 1][..., 2] *= meshes[s - 1].getLZ() / meshes[s].getLZ()
             ps[s - 1] = op.forwardMatrixField(ps[s])
+        self._rec = None
         if rec is not None:
             self.stats = _stats_dict(rec)
         return us[0], ps[0]
