@@ -183,11 +183,15 @@ __global__ __launch_bounds__(kBlock) void k_init_state(const LevelDesc* __restri
   for (int k = 0; k < 9; ++k) d.p[0][k * N + i] = from_input ? d.p_in[9 * i + k] : 0.0f;
 }
 
-struct Float4Fetch {
-  const float4* v;
-  int NY, NX;
+// The linearisation's taps of J through a buffer descriptor: 32-bit byte
+// offsets (16 N < 2^32 by the plan's size limit), whose plane and row terms
+// the compiler keeps per z-plane and per row, so a tap costs one add instead
+// of a 64-bit index and address computation.
+struct Float4BufFetch {
+  svs_rsrc r;
+  uint32_t sy, sz;  // bytes per row and per plane
   __device__ void operator()(int z, int y, int x, float* o) const {
-    const float4 t = v[((int64_t)z * NY + y) * NX + x];
+    const svs_f4 t = bld4(r, (uint32_t)z * sz + (uint32_t)y * sy + (uint32_t)x * 16u, 0);
     o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
   }
 };
@@ -196,8 +200,10 @@ struct Float4Fetch {
 // I1w = warp(I1, u), g = warp(grad I1, u), rho_c = I1w - sum(u * g) - I0.
 // The four fields share the 64-tap geometry; each is evaluated exactly as the
 // scalar / vector-field warps do it.
-// Five waves per SIMD: 94 instead of 97 VGPRs, no spills (68 x 80^3 batch
-// 612.8 -> 611.1 ms; six waves, 80 VGPRs, is no faster).
+// Five waves per SIMD (85 VGPRs, no spills; six waves at 80 VGPRs are no
+// faster).  The arithmetic bounds it: the 84 Catmull-Rom evaluations of a
+// voxel (21 per field, 21 VALU instructions each) are about three quarters
+// of its ~2,300 VALU instructions per wave.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_linearise(const LevelDesc* __restrict__ descs, const int* __restrict__ block_start, int ninst,
                             int parity, int ip, int bc) {
   SVS_BATCH_PROLOGUE(descs);
@@ -208,8 +214,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
   const float cy = (float)y * d.hY + u1;
   const float cz = (float)z * d.hZ + u2;
   float v[4];
-  sample<4, Float4Fetch, SVS_LIN_LOOP>(ip, Float4Fetch{d.J, d.NY, d.NX}, d.NZ, d.NY, d.NX, d.hZ, d.hY, d.hX, bc, cz, cy,
-                                       cx, v);
+  const uint32_t sy = (uint32_t)d.NX * 16u;
+  const Float4BufFetch J{make_rsrc(d.J, (uint32_t)N * 16u), sy, (uint32_t)d.NY * sy};
+  sample<4, Float4BufFetch, SVS_LIN_LOOP>(ip, J, d.NZ, d.NY, d.NX, d.hZ, d.hY, d.hX, bc, cz, cy, cx, v);
   const float s = (u0 * v[1] + u2 * v[3]) + u1 * v[2];  // torch's sum over dim 3: (a + c) + b
   const float rc = (v[0] - s) - d.I0[i];
   d.RG[i] = make_float4(rc, v[1], v[2], v[3]);
